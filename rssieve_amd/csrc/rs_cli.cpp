@@ -1,0 +1,510 @@
+// rs_cli.cpp -- the 'rssieve' command line: the reference's -p/-P/-i/-f
+// interface (main.cpp:25-134) and sieve driver (cl_sieve.cpp:975-1451) on top
+// of the C ABI of librssieve.so.  Same ABCD input, factor-file lines,
+// checkpoint files and summary line as the reference.
+#include <algorithm>
+#include <cerrno>
+#include <chrono>
+#include <cinttypes>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+#include <vector>
+
+#include "rssieve.h"
+#include "rssieve_io.h"
+
+namespace {
+
+struct Options {
+  uint64_t pmin = 0, pmax = 0;
+  std::string input, factor_file;
+  bool selftest = false;
+  int device = 0;
+  bool merged = false;  // -t: the merged-table BSGS
+  bool compat = false;  // -c: reference signed-int k encoding
+};
+
+void help() {
+  printf("rssieve: a HIP (gfx950) Riesel Sierpinski sieve for multiple sequences k*b^n+-1 using the BSGS algorithm.\n");
+  printf("Program usage:\n");
+  printf("-p #\t\t\tStarting prime factor p\n");
+  printf("-P #\t\t\tEnd prime factor P\n");
+  printf("\t\t\tP range is 3 <= -p < -P < 2^64, [-p, -P) exclusive\n");
+  printf("-i inputfile\t\tUse specified sr2sieve ABCD input file with a maximum of 100 sequences\n");
+  printf("-f factorfile\t\tOverride default factor file name (psp_sr2sieve.out) with specified file name.\n");
+  printf("-d #\t\t\tHIP device index (default 0)\n");
+  printf("-t\t\t\tUse the merged-table BSGS with tuned parameters\n");
+  printf("-c\t\t\tEncode k as the reference's signed int (k >= 2^31 wraps)\n");
+  printf("-s\t\t\tRun the built-in self test\n");
+  printf("-h\t\t\tPrint this help\n");
+  exit(EXIT_FAILURE);
+}
+
+// main.cpp:39-70: unsigned 64-bit value with an optional e<exp> suffix
+int parse_uint64(uint64_t* result, const char* str, uint64_t lo, uint64_t hi) {
+  if (!str || !*str) return -1;
+  char* tail;
+  errno = 0;
+  uint64_t num = strtoull(str, &tail, 10);
+  if (errno != 0) return -2;
+  if (*tail == 'e' || *tail == 'E') {
+    tail++;
+    errno = 0;
+    uint64_t exp = strtoul(tail, &tail, 10);
+    if (errno != 0) return -2;
+    if (*tail != '\0') return -1;
+    while (exp-- > 0) {
+      if (num > hi / 10) return -2;
+      num *= 10;
+    }
+  } else if (*tail != '\0') {
+    return -1;
+  }
+  if (num < lo || num > hi) return -2;
+  *result = num;
+  return 0;
+}
+
+// main.cpp:83-134: whitespace-split tokens; unknown tokens are ignored
+void parse_cmdline(int argc, char** argv, Options* o) {
+  std::string joined;
+  for (int i = 1; i < argc; i++) {
+    joined += argv[i];
+    if (i < argc - 1) joined += " ";
+  }
+  std::vector<char> buf(joined.begin(), joined.end());
+  buf.push_back('\0');
+  char* save = nullptr;
+  char* tok = strtok_r(buf.data(), " \t", &save);
+  while (tok) {
+    if (!strcmp(tok, "-p")) {
+      tok = strtok_r(nullptr, " \t", &save);
+      if (tok && parse_uint64(&o->pmin, tok, 3, 0xFFFFFFFFFFFFFFFFull) != 0) {
+        fprintf(stderr, "Invalid value for -p: %s\n", tok);
+        printf("Invalid value for -p: %s\n", tok);
+      }
+    } else if (!strcmp(tok, "-P")) {
+      tok = strtok_r(nullptr, " \t", &save);
+      if (tok && parse_uint64(&o->pmax, tok, 4, 0xFFFFFFFFFFFFFFFFull) != 0) {
+        fprintf(stderr, "Invalid value for -P: %s\n", tok);
+        printf("Invalid value for -P: %s\n", tok);
+      }
+    } else if (!strcmp(tok, "-i")) {
+      tok = strtok_r(nullptr, " \t", &save);
+      if (tok) o->input = tok;
+    } else if (!strcmp(tok, "-f")) {
+      tok = strtok_r(nullptr, " \t", &save);
+      if (tok) o->factor_file = tok;
+    } else if (!strcmp(tok, "-d")) {
+      tok = strtok_r(nullptr, " \t", &save);
+      if (tok) o->device = atoi(tok);
+    } else if (!strcmp(tok, "-s")) {
+      o->selftest = true;
+    } else if (!strcmp(tok, "-t")) {
+      o->merged = true;
+    } else if (!strcmp(tok, "-c")) {
+      o->compat = true;
+    } else if (!strcmp(tok, "-h")) {
+      help();
+    }
+    if (!tok) break;
+    tok = strtok_r(nullptr, " \t", &save);
+  }
+}
+
+// cl_sieve.cpp:646-659
+bool is_perfect_power(uint64_t b, uint32_t* base) {
+  for (uint64_t a = 2; a * a <= b; ++a) {
+    uint64_t p = a * a;
+    while (p < b) p *= a;
+    if (p == b) {
+      *base = (uint32_t)a;
+      return true;
+    }
+  }
+  return false;
+}
+
+[[noreturn]] void die(const char* what) {
+  fprintf(stderr, "%s\n", what);
+  printf("%s\n", what);
+  exit(EXIT_FAILURE);
+}
+
+[[noreturn]] void die_lib(const char* ctx_msg) {
+  std::string m = std::string(ctx_msg) + ": " + rs_last_error();
+  die(m.c_str());
+}
+
+void format_rate(double val, char* buf) {
+  const char* units[] = {"", "k", "M", "G", "T", "P"};
+  int u = 0;
+  while (val >= 1000.0 && u < 5) {
+    val /= 1000.0;
+    u++;
+  }
+  sprintf(buf, "%6.2f %sp/s", val, units[u]);
+}
+
+void format_eta(double seconds, char* buf) {
+  int h = (int)(seconds / 3600);
+  int m = ((int)seconds % 3600) / 60;
+  int s = (int)seconds % 60;
+  sprintf(buf, "%02d:%02d:%02d", h, m, s);
+}
+
+// cl_sieve.cpp:86-144: progress bar with an exponentially smoothed p/s
+void print_progress(const rs_state& st, double* smooth, time_t start_time, uint64_t run_start_p) {
+  const int bar_width = 40;
+  const uint64_t total = st.pmax - st.pmin, done = st.p - st.pmin;
+  double progress = total ? (double)done / (double)total : 0.0;
+  progress = std::min(1.0, std::max(0.0, progress));
+  const int pos = (int)(progress * bar_width);
+  const double elapsed = difftime(time(nullptr), start_time);
+  const uint64_t done_run = st.p > run_start_p ? st.p - run_start_p : 0;
+  const double inst = (double)done_run / (elapsed > 0 ? elapsed : 1.0);
+  *smooth = (*smooth == 0.0) ? inst : 0.9 * (*smooth) + 0.1 * inst;
+  const uint64_t remaining = st.pmax > st.p ? st.pmax - st.p : 0;
+  const double remain = (double)remaining / (*smooth > 0.0 ? *smooth : 1.0);
+  char rate[32], eta[32];
+  format_rate(*smooth, rate);
+  format_eta(remain, eta);
+  printf("\r[");
+  for (int i = 0; i < bar_width; i++) putchar(i < pos ? '#' : '-');
+  printf("] %6.2f%% | %s | ETA %s", progress * 100.0, rate, eta);
+  fflush(stdout);
+}
+
+struct Run {
+  Options o;
+  rs_abcd* abcd = nullptr;
+  rs_ctx* ctx = nullptr;
+  rs_job* job = nullptr;
+  rs_state st{};
+  int write_a_next = 1;
+  uint64_t factor_cap = 0;
+  uint64_t base_primecount = 0, base_dups = 0;  // totals restored from a checkpoint
+  std::vector<rs_factor> fbuf;
+};
+
+// cl_sieve.cpp:530-643: drain, sort by (p, n), verify on the CPU, append
+void get_results(Run& r, bool print) {
+  if (print) {
+    printf("\r                                                                                \r");
+    fflush(stdout);
+  }
+  uint64_t n = 0;
+  int rc = rs_factors_drain(r.job, r.fbuf.data(), r.fbuf.size(), &n);
+  if (rc == RS_E_OVERFLOW) {
+    fprintf(stderr, "Error: %s\n", rs_last_error());
+    exit(EXIT_FAILURE);
+  }
+  if (rc) die_lib("rs_factors_drain");
+  rs_counters c;
+  if (rs_counters_read(r.job, &c)) die_lib("rs_counters_read");
+  if (c.seg_overflow) die("error: gpu prime array overflow");
+  r.st.primecount = r.base_primecount + c.primecount;
+  r.st.dupcount = r.base_dups + c.dups;
+  if (n) {
+    std::sort(r.fbuf.begin(), r.fbuf.begin() + n, [](const rs_factor& a, const rs_factor& b) {
+      if (a.p != b.p) return a.p < b.p;
+      if (a.n != b.n) return a.n < b.n;
+      return a.kidx < b.kidx;
+    });
+    if (print) printf("Verifying %" PRIu64 " factors on CPU...\n", n);
+    uint32_t prp = 0;
+    FILE* res = nullptr;
+    for (uint64_t i = 0; i < n; i++) {
+      const rs_factor& f = r.fbuf[i];
+      uint32_t k;
+      int32_t cc;
+      rs_job_sequence(r.job, f.kidx, &k, &cc);
+      const int v = rs_verify_factor(f.p, k, f.n, cc, r.st.base);
+      if (v == 0) {
+        fprintf(stderr, "CPU factor verification failed!  %" PRIu64 " is not a factor of %u*%u^%u%+d\n", f.p, k,
+                r.st.base, f.n, cc);
+        printf("CPU factor verification failed!  %" PRIu64 " is not a factor of %u*%u^%u%+d\n", f.p, k, r.st.base,
+               f.n, cc);
+        exit(EXIT_FAILURE);
+      } else if (v == -1) {
+        ++prp;
+      } else {
+        ++r.st.factorcount;
+        if (!res) {
+          res = fopen(r.o.factor_file.c_str(), "a");
+          if (!res) {
+            fprintf(stderr, "Cannot open %s !!!\n", r.o.factor_file.c_str());
+            exit(EXIT_FAILURE);
+          }
+        }
+        char line[128];
+        rs_format_factor(line, sizeof(line), f.p, k, r.st.base, f.n, cc);
+        if (fputs(line, res) < 0) {
+          fprintf(stderr, "Cannot write to %s !!!\n", r.o.factor_file.c_str());
+          exit(EXIT_FAILURE);
+        }
+        r.st.checksum += f.p + k + f.n + (uint64_t)(int64_t)cc;
+      }
+    }
+    if (res) fclose(res);
+    fprintf(stderr, "Verified %" PRIu64 " factors.\n", n - prp);
+    if (print) printf("Verified %" PRIu64 " factors.\n", n - prp);
+  }
+  // checkpoint (cl_sieve.cpp:321-328)
+  if (rs_state_write(".", &r.st, &r.write_a_next)) fprintf(stderr, "%s\n", rs_last_error());
+  if (print) printf("Checkpoint, current p: %" PRIu64 "\n", r.st.p);
+}
+
+// cl_sieve.cpp:797-863
+void read_factors_update_bitmap(Run& r) {
+  FILE* fp = fopen(r.o.factor_file.c_str(), "r");
+  if (!fp) {
+    fprintf(stderr, "Cannot open %s !!!\n", r.o.factor_file.c_str());
+    printf("Cannot open %s !!!\n", r.o.factor_file.c_str());
+    exit(EXIT_FAILURE);
+  }
+  char line[512];
+  uint64_t ln = 0;
+  while (fgets(line, sizeof(line), fp)) {
+    ln++;
+    uint64_t p;
+    uint32_t k, n;
+    char sign;
+    if (rs_parse_factor(line, r.st.base, &p, &k, &sign, &n)) {
+      fprintf(stderr, "Invalid line read in factors file %" PRIu64 ": %s", ln, line);
+      printf("Invalid line read in factors file %" PRIu64 ": %s", ln, line);
+      exit(EXIT_FAILURE);
+    }
+    rs_abcd_mark_used(r.abcd, k, sign, n, r.o.compat ? 1 : 0);
+  }
+  fclose(fp);
+  if (ln < r.st.factorcount) {
+    fprintf(stderr, "ERROR: Missing factors in %s !!!\n", r.o.factor_file.c_str());
+    printf("ERROR: Missing factors in %s !!!\n", r.o.factor_file.c_str());
+    exit(EXIT_FAILURE);
+  }
+  printf("Updated bitmap with %" PRIu64 " factors from %s\n", ln, r.o.factor_file.c_str());
+}
+
+// cl_sieve.cpp:738-795
+void finalize(Run& r) {
+  if (r.st.factorcount) {
+    FILE* f = fopen(r.o.factor_file.c_str(), "r");
+    if (!f) {
+      fprintf(stderr, "Cannot open %s !!!\n", r.o.factor_file.c_str());
+      exit(EXIT_FAILURE);
+    }
+    char line[256];
+    uint64_t lc = 0;
+    while (fgets(line, sizeof(line), f)) ++lc;
+    fclose(f);
+    if (lc < r.st.factorcount) {
+      fprintf(stderr, "ERROR: Missing factors in %s !!!\n", r.o.factor_file.c_str());
+      printf("ERROR: Missing factors in %s !!!\n", r.o.factor_file.c_str());
+      exit(EXIT_FAILURE);
+    }
+  }
+  FILE* f = fopen(r.o.factor_file.c_str(), "a");
+  if (!f) {
+    fprintf(stderr, "Cannot open %s !!!\n", r.o.factor_file.c_str());
+    exit(EXIT_FAILURE);
+  }
+  if (!r.st.factorcount && fprintf(f, "no factors\n") < 0) {
+    fprintf(stderr, "Cannot write to %s !!!\n", r.o.factor_file.c_str());
+    exit(EXIT_FAILURE);
+  }
+  fclose(f);
+}
+
+int sieve(Options o) {
+  Run r;
+  r.o = o;
+  if (o.input.empty()) die("ERROR: no input file specified!");
+  if (rs_abcd_read(o.input.c_str(), &r.abcd)) {
+    fprintf(stderr, "%s\n", rs_last_error());
+    exit(EXIT_FAILURE);
+  }
+  uint32_t base, nmin, nmax;
+  int32_t kcount;
+  uint64_t wps, nterms;
+  rs_abcd_info(r.abcd, &base, &kcount, &nmin, &nmax, &wps, &nterms);
+  printf("Sequences read: %d\n", kcount);
+  fprintf(stderr, "Sequences read: %d\n", kcount);
+  printf("Base = %" PRIu32 "\n", base);
+  fprintf(stderr, "Base = %" PRIu32 "\n", base);
+  if (r.o.factor_file.empty()) r.o.factor_file = "psp_sr2sieve.out";  // cl_sieve.cpp:992-994
+
+  // setupSearch (cl_sieve.cpp:661-689)
+  uint32_t bp;
+  if (is_perfect_power(base, &bp)) {
+    printf("\nerror: selected base %u is a perfect power of base %u!\n", base, bp);
+    fprintf(stderr, "error: selected base %u is a perfect power of base %u!\n", base, bp);
+    exit(EXIT_FAILURE);
+  }
+  if (!o.pmin || !o.pmax) {
+    printf("\n-p and -P arguments are required\nuse -h for help\n");
+    fprintf(stderr, "-p and -P arguments are required\n");
+    exit(EXIT_FAILURE);
+  }
+  if (o.pmin > o.pmax) {
+    printf("pmin <= pmax is required\nuse -h for help\n");
+    fprintf(stderr, "pmin <= pmax is required\n");
+    exit(EXIT_FAILURE);
+  }
+  rs_state& st = r.st;
+  st.pmin = o.pmin;
+  st.pmax = o.pmax;
+  st.p = o.pmin;
+  st.base = base;
+  st.nmin = nmin;
+  st.nmax = nmax;
+  st.kcount = kcount;
+  std::vector<uint32_t> ks(kcount);
+  std::vector<int32_t> cs(kcount);
+  rs_abcd_sequences(r.abcd, ks.data(), cs.data());
+  for (int i = 0; i < kcount; i++) st.klist[i] = (cs[i] > 0) ? (int32_t)ks[i] : (int32_t)(0u - ks[i]);
+  fprintf(stderr, "Starting sieve at p: %" PRIu64 " n: %u\nStopping sieve at P: %" PRIu64 " N: %u\n", st.pmin,
+          st.nmin, st.pmax, st.nmax);
+  printf("Starting sieve at p: %" PRIu64 " n: %u\nStopping sieve at P: %" PRIu64 " N: %u\n", st.pmin, st.nmin,
+         st.pmax, st.nmax);
+
+  if (rs_open(o.device, &r.ctx)) die_lib("rs_open");
+  uint32_t cus, lds;
+  char name[256];
+  rs_device_info(r.ctx, &cus, &lds, name, sizeof(name));
+  printf("GPU Info:\n  Name: \t\t%s\n  Compute Units: \t%u\n  Local Mem Size: \t%u bytes\n", name, cus, lds);
+  fprintf(stderr, "GPU Info:\n  Name: \t\t%s\n  Compute Units: \t%u\n  Local Mem Size: \t%u bytes\n", name, cus, lds);
+
+  // resume or start (cl_sieve.cpp:1035-1074)
+  const int resumed = rs_state_read(".", &st, &r.write_a_next);
+  if (resumed > 0) {
+    printf("Resuming from checkpoint in %s\n", r.write_a_next ? "stateB.ckp" : "stateA.ckp");
+    printf("Current p: %" PRIu64 "\n", st.p);
+    fprintf(stderr, "Resuming from checkpoint, current p: %" PRIu64 "\n", st.p);
+    if (st.p == st.pmax) {
+      printf("Workunit complete.\n");
+      fprintf(stderr, "Workunit complete.\n");
+      return EXIT_SUCCESS;
+    }
+    read_factors_update_bitmap(r);
+  } else {
+    FILE* t = fopen(r.o.factor_file.c_str(), "w");
+    if (!t) {
+      fprintf(stderr, "Cannot open %s !!!\n", r.o.factor_file.c_str());
+      exit(EXIT_FAILURE);
+    }
+    fclose(t);
+    st.last_trickle = (uint64_t)time(nullptr);
+  }
+  r.base_primecount = st.primecount;
+  r.base_dups = st.dupcount;
+
+  rs_params prm;
+  if (rs_params_reference(nmin, nmax, kcount, 65536, cus, &prm)) die_lib("rs_params_reference");
+  prm.k31_compat = o.compat ? 1 : 0;
+  prm.factor_cap = 10000000ull;                    // main.cpp:167
+  if (st.p < UINT32_MAX) prm.factor_cap *= 10;     // cl_sieve.cpp:1081-1083
+  r.factor_cap = prm.factor_cap;
+  if (rs_job_create(r.ctx, base, kcount, ks.data(), cs.data(), nmin, nmax, rs_abcd_bitmap(r.abcd), wps, &prm,
+                    &r.job))
+    die_lib("rs_job_create");
+  uint32_t M, QQ, MM;
+  rs_params_derive(&prm, nmin, nmax, &M, &QQ, &MM);
+  printf("Q:%u M:%u QQ:%u MM:%u HT:%u LS:%u\n", prm.Q, M, QQ, MM, prm.hsize, prm.threads);
+  fprintf(stderr, "Q:%u M:%u QQ:%u MM:%u HT:%u LS:%u\n", prm.Q, M, QQ, MM, prm.hsize, prm.threads);
+  r.fbuf.resize(r.factor_cap);
+
+  const time_t start_time = time(nullptr);
+  time_t last_prog = start_time, last_ckpt = start_time;
+  double smooth = 0;
+  const uint64_t run_start_p = st.p;
+  fprintf(stderr, "Starting Sieve...\n");
+  printf("Starting Sieve...\n");
+  const auto t0 = std::chrono::steady_clock::now();
+  // Chunks of the p-range between host checks; the width adapts so that a
+  // chunk takes about a second and cannot fill the factor list.
+  uint64_t chunk = 1u << 20;
+  while (st.p < st.pmax) {
+    uint64_t stop = st.p + chunk;
+    if (stop > st.pmax || stop < st.p) stop = st.pmax;
+    const auto c0 = std::chrono::steady_clock::now();
+    if (rs_sieve(r.job, st.p, stop)) die_lib("rs_sieve");
+    rs_counters c;
+    if (rs_counters_read(r.job, &c)) die_lib("rs_counters_read");
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
+    st.p = stop;
+    if (dt < 0.5 && c.nfactors < r.factor_cap / 8 && chunk < (1ull << 62)) chunk *= 2;
+    else if (dt > 2.0 && chunk > (1u << 16)) chunk /= 2;
+    const time_t now = time(nullptr);
+    if (now - last_prog > 1) {
+      print_progress(st, &smooth, start_time, run_start_p);
+      last_prog = now;
+    }
+    if (now - last_ckpt > 60 || c.nfactors > r.factor_cap / 2) {
+      get_results(r, true);
+      last_ckpt = now;
+    }
+  }
+  st.p = st.pmax;
+  get_results(r, true);
+  finalize(r);
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fprintf(stderr, "Sieve complete.\nfactors %" PRIu64 ", prime count %" PRIu64 ", duplicate factors %" PRIu64 "\n",
+          st.factorcount, st.primecount, st.dupcount);
+  printf("Sieve finished in %d sec.\n", (int)secs);
+  printf("factors %" PRIu64 ", prime count %" PRIu64 ", checksum %016" PRIX64 ", duplicate factors %" PRIu64 "\n",
+         st.factorcount, st.primecount, st.checksum, st.dupcount);
+  rs_job_destroy(r.job);
+  rs_close(r.ctx);
+  rs_abcd_free(r.abcd);
+  return EXIT_SUCCESS;
+}
+
+// Built-in self test (the reference's -s is a stub, cl_sieve.cpp:1462-1469):
+// the known factor 1000133 | 91549*2^54378870+1 (cl_sieve.cpp:826-827) must be
+// found, with every reported factor exact.
+int selftest(int device) {
+  rs_ctx* ctx;
+  if (rs_open(device, &ctx)) die_lib("rs_open");
+  const uint32_t k = 91549;
+  const int32_t c = 1;
+  const uint32_t n0 = 54378870, nmin = n0 - 1000, nmax = n0 + 1000;  // nmin even
+  const uint64_t wps = ((uint64_t)nmax - nmin + 1 + 31) / 32;
+  std::vector<uint32_t> bm(wps, 0xFFFFFFFFu);
+  rs_params prm;
+  rs_params_reference(nmin, nmax, 1, 65536, 1, &prm);
+  rs_job* job;
+  if (rs_job_create(ctx, 2, 1, &k, &c, nmin, nmax, bm.data(), wps, &prm, &job)) die_lib("rs_job_create");
+  if (rs_sieve(job, 1000000, 1001000)) die_lib("rs_sieve");
+  std::vector<rs_factor> f(1 << 16);
+  uint64_t n = 0;
+  if (rs_factors_drain(job, f.data(), f.size(), &n)) die_lib("rs_factors_drain");
+  bool found = false, exact = true;
+  for (uint64_t i = 0; i < n; i++) {
+    if (f[i].p == 1000133 && f[i].n == n0) found = true;
+    if (rs_verify_factor(f[i].p, k, f[i].n, c, 2) != 1) exact = false;
+  }
+  rs_job_destroy(job);
+  rs_close(ctx);
+  printf("self test: %" PRIu64 " factors, 1000133 | 91549*2^54378870+1 %s, all exact: %s\n", n,
+         found ? "found" : "MISSING", exact ? "yes" : "NO");
+  return (found && exact) ? EXIT_SUCCESS : EXIT_FAILURE;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  fprintf(stderr, "\nrssieve (HIP, gfx950)\n");
+  printf("\nrssieve (HIP, gfx950)\n");
+  fprintf(stderr, "Command line: ");
+  for (int i = 0; i < argc; i++) fprintf(stderr, "%s ", argv[i]);
+  fprintf(stderr, "\n");
+  Options o;
+  parse_cmdline(argc, argv, &o);
+  if (o.selftest) return selftest(o.device);
+  return sieve(o);
+}

@@ -1,0 +1,77 @@
+// rs_internal.h -- device data layouts shared by the HIP kernels and the
+// host side of the C-ABI library.  Not part of the public interface.
+#pragma once
+#include <stdint.h>
+
+#define RS_MAXK 100          // sequences per job (reference cl_sieve.h:2)
+#define RS_NCLASS 3          // k classes with work: full n range, even n, odd n
+
+namespace rs {
+
+// Per-job constants, resident in device memory and read with uniform loads.
+struct JobDev {
+  uint32_t base;
+  int32_t kcount;
+  uint32_t nmin, nmax;        // bitmap NMIN (even) and NMAX
+  uint64_t words_per_seq;
+  uint64_t ak[RS_MAXK];       // |k| fed to the Montgomery setup (setup.cl:143)
+  int32_t cpos[RS_MAXK];      // 1 if the target is -k^-1 (c = +1), setup.cl:188
+  // BSGS shape
+  uint32_t algo;              // 0: three lists (reference shape), 1: one merged table
+  uint32_t Q, M, QQ, MM;      // baby steps / giant steps, full and parity lists
+  uint32_t hsize, hmask;      // LDS hash slots (power of 2)
+  uint32_t threads;           // BSGS workgroup size
+  uint32_t record_factors;    // 0: claims clear bitmap bits only
+  uint64_t factor_cap;
+};
+
+// Per-prime record written by the setup kernel (128 bytes).
+struct PrimeRec {
+  uint64_t p, q, one, mbase;
+  uint64_t bn;                // b^NMIN (Montgomery)
+  uint64_t g[2];              // giant step: b^-Q (full) and b^-QQ (parity)
+  uint64_t gT[2];             // g[c]^threads
+  uint64_t bT[2];             // b^threads and b^(2*threads): baby stride per round
+  uint32_t nk[RS_NCLASS];     // k count per class: full, even, odd (0 = skipped)
+  uint32_t flags;             // reserved
+  uint64_t pad[3];
+};
+static_assert(sizeof(PrimeRec) == 128, "PrimeRec layout");
+
+// Per (prime, k) entry, grouped by class: full, then even, then odd.
+struct KRec {
+  uint64_t hadj;              // target b^n: k^-1 (c=-1) or -k^-1 (c=+1), Montgomery
+  uint32_t kidx;              // sequence index
+  uint32_t cls;               // 1 full, 2 even, 3 odd (the reference's parity code)
+};
+
+// Factor record handed to the host (p, n, sequence index).
+struct FactorRec {
+  uint64_t p;
+  uint32_t n;
+  uint32_t kidx;
+};
+
+// Device counters of one job (read by the host at checkpoints).
+struct Counters {
+  uint64_t primecount;        // 2-PRPs generated
+  uint64_t claims;            // successful bitmap claims
+  uint64_t dups;              // failed claims
+  uint64_t nfactors;          // factor records appended (may exceed cap)
+  uint64_t skipped;           // primes dropped by setup
+  uint64_t lists[RS_NCLASS];  // (prime, class) items with work
+  uint64_t order_lists;       // items solved by the order shortcut
+  uint64_t work_mmul;         // Montgomery products of the work model
+  uint64_t work_probe;        // hash lookups
+  uint64_t max_seg_primes;    // largest per-segment prime count (overflow check)
+  uint64_t seg_overflow;      // segments whose primes exceeded the buffer
+};
+
+// Per-segment scheduling state (reset by a kernel at the start of a segment).
+struct SegState {
+  uint32_t nprimes;           // primes written by the generator
+  uint32_t work_next;         // BSGS work-queue head
+  uint32_t pad[2];
+};
+
+}  // namespace rs
