@@ -138,9 +138,11 @@ int rs_factors_drain(rs_job *job, rs_factor *out, uint64_t cap, uint64_t *n);
 /* Read / replace the device bitmap (cl_sieve.cpp:1266). */
 int rs_bitmap_read(rs_job *job, uint32_t *out, uint64_t nwords);
 int rs_bitmap_write(rs_job *job, const uint32_t *in, uint64_t nwords);
-/* Seconds the last rs_sieve/rs_sieve_batch call kept the BSGS kernel busy,
- * from HIP events around every BSGS launch; and the number of launches. */
+/* Seconds the BSGS kernel ran since the last rs_job_timer_reset (HIP events
+ * around every BSGS launch on the context stream), the number of launches,
+ * and the stream time from the first to the last sieve call; synchronises. */
 int rs_job_kernel_time(rs_job *job, double *bsgs_seconds, uint64_t *bsgs_launches, double *total_seconds);
+int rs_job_timer_reset(rs_job *job);
 
 /* --- stage entry points (parity tests) -------------------------------- */
 

@@ -105,6 +105,7 @@ _SIGS = {
     "rs_bitmap_write": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint64]),
     "rs_job_kernel_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_double)]),
+    "rs_job_timer_reset": (C.c_int, [C.c_void_p]),
     "rs_primegen": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64), C.c_uint64,
                               C.POINTER(C.c_uint64)]),
     "rs_verify_factor": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int32, C.c_uint32]),
@@ -355,6 +356,9 @@ class Job:
         t, n, tot = C.c_double(), C.c_uint64(), C.c_double()
         _check(lib().rs_job_kernel_time(self._h, C.byref(t), C.byref(n), C.byref(tot)))
         return t.value, n.value, tot.value
+
+    def timer_reset(self):
+        _check(lib().rs_job_timer_reset(self._h))
 
     def close(self):
         if self._h:

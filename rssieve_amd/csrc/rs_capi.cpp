@@ -15,17 +15,14 @@
 #include "rs_internal.h"
 
 namespace rs {
-hipError_t launch_seg_begin(SegState* seg, hipStream_t s);
-hipError_t launch_smallprimes(uint64_t low, uint64_t high, uint64_t* out, uint32_t cap, SegState* seg, hipStream_t s);
-hipError_t launch_primegen(uint64_t low, uint64_t high, int32_t widx, uint64_t* out, uint32_t cap, SegState* seg,
-                           hipStream_t s);
-hipError_t launch_setup(const JobDev* job, const uint64_t* primes, const SegState* seg, PrimeRec* recs, KRec* krecs,
-                        Counters* cnt, uint32_t cap, hipStream_t s);
 hipError_t bsgs_blocks_per_cu(uint32_t threads, uint32_t hsize, int* out);
-hipError_t launch_bsgs(uint32_t threads, uint32_t hsize, uint32_t nblocks, const JobDev* job, const PrimeRec* recs,
-                       const KRec* krecs, SegState* seg, uint32_t* bitmap, FactorRec* fac, Counters* cnt,
-                       uint32_t cap, hipStream_t s);
 size_t bsgs_lds_bytes(uint32_t hsize);
+hipError_t launch_set_cursor(SegJob* sj, uint64_t pmin, uint64_t pmax, uint32_t threads, uint64_t range,
+                             uint32_t gen_blocks, hipStream_t s);
+hipError_t launch_round(SegJob* jobs, BatchState* bs, uint32_t njobs, uint32_t gen_blocks, uint32_t setup_blocks,
+                        uint32_t threads, uint32_t hsize, uint32_t bsgs_blocks, hipEvent_t ev0, hipEvent_t ev1,
+                        hipStream_t s);
+hipError_t launch_primegen_round(SegJob* jobs, BatchState* bs, uint32_t gen_blocks, hipStream_t s);
 }  // namespace rs
 
 using namespace rs;
@@ -53,12 +50,25 @@ static int fail(int code, const char* fmt, ...) {
                                       __FILE__, __LINE__);                                    \
   } while (0)
 
+struct GraphKey {
+  uint32_t njobs, gen_blocks, setup_blocks, threads, hsize, bsgs_blocks;
+  bool operator==(const GraphKey& o) const {
+    return njobs == o.njobs && gen_blocks == o.gen_blocks && setup_blocks == o.setup_blocks && threads == o.threads &&
+           hsize == o.hsize && bsgs_blocks == o.bsgs_blocks;
+  }
+};
+
 struct rs_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   uint32_t cus = 0;
   uint32_t lds_per_block = 0;
   char name[256] = {0};
+  // batch staging: SegJob array and work queue shared by rs_sieve_batch calls
+  SegJob* d_bjobs = nullptr;
+  BatchState* d_bstate = nullptr;
+  // one captured round per batch shape
+  std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
 };
 
 struct rs_job {
@@ -76,7 +86,9 @@ struct rs_job {
   uint32_t* dbitmap = nullptr;
   Counters* dcnt = nullptr;
   FactorRec* dfac = nullptr;
-  SegState* dseg = nullptr;
+  SegJob* d_sj = nullptr;       // this job as a batch of one
+  BatchState* d_bs = nullptr;
+  SegJob h_sj{};                // pointers of d_sj (cursor fields are set on the device)
   uint64_t* dprimes = nullptr;
   PrimeRec* drecs = nullptr;
   KRec* dkrecs = nullptr;
@@ -85,6 +97,7 @@ struct rs_job {
   size_t ev_used = 0;
   hipEvent_t ev_all[2] = {nullptr, nullptr};
   bool timed = false;
+  bool timer_started = false;
 };
 
 extern "C" const char* rs_last_error(void) { return g_err.c_str(); }
@@ -107,6 +120,13 @@ extern "C" int rs_open(int device, rs_ctx** out) {
     delete c;
     return fail(RS_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
   }
+  if (hipMalloc((void**)&c->d_bjobs, sizeof(SegJob) * RS_MAX_BATCH) != hipSuccess ||
+      hipMalloc((void**)&c->d_bstate, sizeof(BatchState)) != hipSuccess) {
+    (void)hipFree(c->d_bjobs);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(RS_E_NOMEM, "rs_open: batch buffers");
+  }
   *out = c;
   return RS_OK;
 }
@@ -118,6 +138,9 @@ extern "C" void rs_close(rs_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
   }
+  for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+  (void)hipFree(ctx->d_bjobs);
+  (void)hipFree(ctx->d_bstate);
   delete ctx;
 }
 
@@ -177,7 +200,8 @@ static void job_free(rs_job* j) {
   (void)hipFree(j->dbitmap);
   (void)hipFree(j->dcnt);
   (void)hipFree(j->dfac);
-  (void)hipFree(j->dseg);
+  (void)hipFree(j->d_sj);
+  (void)hipFree(j->d_bs);
   (void)hipFree(j->dprimes);
   (void)hipFree(j->drecs);
   (void)hipFree(j->dkrecs);
@@ -206,12 +230,16 @@ extern "C" int rs_job_create(rs_ctx* ctx, uint32_t base, int32_t kcount, const u
   if (nmin & 1u) return fail(RS_E_ARG, "rs_job_create: nmin must be even (bitmap origin)");
   if (words_per_seq < (((uint64_t)nmax - nmin + 1) + 31) / 32)
     return fail(RS_E_ARG, "rs_job_create: words_per_seq too small for [%u, %u]", nmin, nmax);
-  if (prm->algo != RS_ALGO_THREE_LISTS) return fail(RS_E_ARG, "rs_job_create: algorithm %d not available", prm->algo);
+  if (prm->algo != RS_ALGO_THREE_LISTS && prm->algo != RS_ALGO_MERGED)
+    return fail(RS_E_ARG, "rs_job_create: unknown algorithm %d", prm->algo);
+  if (prm->algo == RS_ALGO_MERGED && (prm->Q & 1))
+    return fail(RS_E_ARG, "rs_job_create: the merged table needs an even exponent span Q (got %u)", prm->Q);
   if (!(prm->threads == 128 || prm->threads == 256 || prm->threads == 512 || prm->threads == 1024))
     return fail(RS_E_ARG, "rs_job_create: threads %u not in {128,256,512,1024}", prm->threads);
-  if (prm->Q < 16 || !pow2(prm->hsize) || prm->hsize <= prm->Q || prm->Q > 32768)
-    return fail(RS_E_ARG, "rs_job_create: need Q in [16, 32768] and a power-of-two hsize > Q (Q=%u hsize=%u)",
-                prm->Q, prm->hsize);
+  const uint32_t entries = prm->algo == RS_ALGO_MERGED ? prm->Q / 2 : prm->Q;  // baby steps in one table
+  if (prm->Q < 16 || !pow2(prm->hsize) || prm->hsize <= entries || prm->Q > 65536)
+    return fail(RS_E_ARG, "rs_job_create: need Q in [16, 65536] and a power-of-two hsize above %u table entries "
+                "(Q=%u hsize=%u)", entries, prm->Q, prm->hsize);
   if (bsgs_lds_bytes(prm->hsize) + 64 > ctx->lds_per_block)
     return fail(RS_E_ARG, "rs_job_create: hash of %u slots needs %zu LDS bytes, device has %u", prm->hsize,
                 bsgs_lds_bytes(prm->hsize), ctx->lds_per_block);
@@ -247,17 +275,21 @@ extern "C" int rs_job_create(rs_ctx* ctx, uint32_t base, int32_t kcount, const u
   }
   h.algo = (uint32_t)prm->algo;
   h.Q = prm->Q;
+  {
+    rs_params ref;
+    if (rs_params_reference(nmin, nmax, kcount, 65536, ctx->cus, &ref) != RS_OK) {
+      job_free(j);
+      return RS_E_ARG;
+    }
+    h.Qref = ref.Q;
+  }
   rs_params_derive(prm, nmin, nmax, &h.M, &h.QQ, &h.MM);
   h.hsize = prm->hsize;
   h.hmask = prm->hsize - 1;
   h.threads = prm->threads;
   h.record_factors = prm->record_factors ? 1 : 0;
   h.factor_cap = prm->factor_cap ? prm->factor_cap : 1;
-  if (h.QQ / 2 >= prm->hsize) {
-    job_free(j);
-    return fail(RS_E_ARG, "rs_job_create: hsize %u too small for %u parity baby steps", prm->hsize, h.QQ / 2);
-  }
-  if (h.QQ > 65536) {
+  if (prm->algo == RS_ALGO_THREE_LISTS && h.QQ > 65536) {
     job_free(j);
     return fail(RS_E_ARG, "rs_job_create: QQ = %u exceeds the 16-bit table index", h.QQ);
   }
@@ -280,16 +312,29 @@ extern "C" int rs_job_create(rs_ctx* ctx, uint32_t base, int32_t kcount, const u
   ALLOC(j->dbitmap, j->nwords * 4);
   ALLOC(j->dcnt, sizeof(Counters));
   ALLOC(j->dfac, h.factor_cap * sizeof(FactorRec));
-  ALLOC(j->dseg, sizeof(SegState));
+  ALLOC(j->d_sj, sizeof(SegJob));
+  ALLOC(j->d_bs, sizeof(BatchState));
   ALLOC(j->dprimes, (size_t)j->seg_cap * 8);
   ALLOC(j->drecs, (size_t)j->seg_cap * sizeof(PrimeRec));
-  ALLOC(j->dkrecs, (size_t)j->seg_cap * kcount * sizeof(KRec));
+  ALLOC(j->dkrecs, (size_t)j->seg_cap * kcount * (prm->algo == RS_ALGO_MERGED ? 2 : 1) * sizeof(KRec));
 #undef ALLOC
+  SegJob& sj = j->h_sj;
+  memset(&sj, 0, sizeof(sj));
+  sj.job = j->djob;
+  sj.recs = j->drecs;
+  sj.krecs = j->dkrecs;
+  sj.primes = j->dprimes;
+  sj.bitmap = j->dbitmap;
+  sj.fac = j->dfac;
+  sj.cnt = j->dcnt;
+  sj.cap = j->seg_cap;
+  sj.threads = prm->threads;
   hipStream_t s = ctx->stream;
   if (hipMemcpyAsync(j->djob, &h, sizeof(JobDev), hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemcpyAsync(j->dbitmap, bitmap, j->nwords * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemsetAsync(j->dcnt, 0, sizeof(Counters), s) != hipSuccess ||
-      hipMemsetAsync(j->dseg, 0, sizeof(SegState), s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+      hipMemcpyAsync(j->d_sj, &sj, sizeof(SegJob), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemsetAsync(j->d_bs, 0, sizeof(BatchState), s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
     job_free(j);
     return fail(RS_E_HIP, "rs_job_create: upload failed");
   }
@@ -333,39 +378,32 @@ static uint64_t segment_range(uint64_t start, uint32_t target) {
   return (uint64_t)r;
 }
 
-extern "C" void rs_wheel_offset(uint64_t* start, int32_t* index);
+// integers per segment and the blocks the generator needs for them
+static void segment_shape(const rs_job* j, uint64_t pmin, uint64_t* range, uint32_t* gen_blocks) {
+  const uint32_t target = j->prm.seg_primes ? j->prm.seg_primes : 65536;
+  uint64_t r = segment_range(pmin, target);
+  // below 2^20 the estimate is loose: bound the segment by what the buffer
+  // holds even if every integer coprime to 30 were prime (8 per 30, + 16)
+  if (pmin < (1ull << 20)) r = std::min<uint64_t>(r, (uint64_t)(j->seg_cap - 64) * 30 / 8);
+  if (r < 60) r = 60;
+  *range = r;
+  *gen_blocks = (uint32_t)(((r / 60 + 1) + 255) / 256);
+}
 
-static int enqueue_segment(rs_job* j, uint64_t p, uint64_t stop) {
-  hipStream_t s = j->ctx->stream;
-  HIPCHK(launch_seg_begin(j->dseg, s));
-  if (p < 114) {
-    const uint64_t stop_sm = stop > 114 ? 114 : stop;
-    HIPCHK(launch_smallprimes(p, stop_sm, j->dprimes, j->seg_cap, j->dseg, s));
-    p = stop_sm;
-  }
-  if (p < stop) {
-    uint64_t ks = p;
-    int32_t widx = 0;
-    rs_wheel_offset(&ks, &widx);
-    if (ks < stop && ks >= p) HIPCHK(launch_primegen(ks, stop, widx, j->dprimes, j->seg_cap, j->dseg, s));
-  }
-  HIPCHK(launch_setup(j->djob, j->dprimes, j->dseg, j->drecs, j->dkrecs, j->dcnt, j->seg_cap, s));
-  if (j->timed) {
-    if (j->ev_used + 2 > j->ev.size()) {
-      for (int i = 0; i < 64; i++) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        j->ev.push_back(e);
-      }
+static uint64_t rounds_for(uint64_t pmin, uint64_t pmax, uint64_t range) {
+  if (pmax <= pmin) return 0;
+  return (pmax - pmin + range - 1) / range;
+}
+
+static int record_event(rs_job* j, hipEvent_t* out) {
+  if (j->ev_used + 1 > j->ev.size()) {
+    for (int i = 0; i < 64; i++) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      j->ev.push_back(e);
     }
-    HIPCHK(hipEventRecord(j->ev[j->ev_used], s));
   }
-  HIPCHK(launch_bsgs(j->h.threads, j->h.hsize, j->bsgs_blocks, j->djob, j->drecs, j->dkrecs, j->dseg, j->dbitmap,
-                     j->dfac, j->dcnt, j->seg_cap, s));
-  if (j->timed) {
-    HIPCHK(hipEventRecord(j->ev[j->ev_used + 1], s));
-    j->ev_used += 2;
-  }
+  *out = j->ev[j->ev_used++];
   return RS_OK;
 }
 
@@ -374,33 +412,97 @@ extern "C" int rs_sieve(rs_job* j, uint64_t pmin, uint64_t pmax) {
   if (pmin < 3) return fail(RS_E_ARG, "rs_sieve: pmin %llu < 3", (unsigned long long)pmin);
   if (pmax < pmin) return fail(RS_E_ARG, "rs_sieve: pmax < pmin");
   HIPCHK(hipSetDevice(j->ctx->device));
-  const uint32_t target = j->prm.seg_primes ? j->prm.seg_primes : 65536;
-  uint64_t range = segment_range(pmin, target);
-  // the buffer holds seg_cap primes; with every integer coprime to 30 a prime
-  // a segment of r integers has at most 8r/30 + 16 of them
-  if (pmin < (1ull << 20)) range = std::min<uint64_t>(range, (uint64_t)(j->seg_cap - 16) * 30 / 8);
-  if (range < 60) range = 60;
-  j->ev_used = 0;
-  j->timed = true;
-  HIPCHK(hipEventRecord(j->ev_all[0], j->ctx->stream));
-  uint64_t p = pmin;
-  while (p < pmax) {
-    uint64_t stop = p + range;
-    if (stop > pmax || stop < p) stop = pmax;
-    int rc = enqueue_segment(j, p, stop);
-    if (rc) return rc;
-    p = stop;
+  hipStream_t s = j->ctx->stream;
+  uint64_t range;
+  uint32_t gen_blocks;
+  segment_shape(j, pmin, &range, &gen_blocks);
+  HIPCHK(launch_set_cursor(j->d_sj, pmin, pmax, j->h.threads, range, gen_blocks, s));
+  if (!j->timer_started) {
+    HIPCHK(hipEventRecord(j->ev_all[0], s));
+    j->timer_started = true;
   }
-  HIPCHK(hipEventRecord(j->ev_all[1], j->ctx->stream));
+  const uint64_t R = rounds_for(pmin, pmax, range);
+  const uint32_t setup_blocks = (j->seg_cap + 3) / 4;
+  for (uint64_t r = 0; r < R; r++) {
+    hipEvent_t e0, e1;
+    int rc = record_event(j, &e0);
+    if (!rc) rc = record_event(j, &e1);
+    if (rc) return rc;
+    HIPCHK(launch_round(j->d_sj, j->d_bs, 1, gen_blocks, setup_blocks, j->h.threads, j->h.hsize, j->bsgs_blocks, e0,
+                        e1, s));
+  }
+  HIPCHK(hipEventRecord(j->ev_all[1], s));
   return RS_OK;
 }
 
 extern "C" int rs_sieve_batch(rs_job** jobs, int njobs, const uint64_t* pmin, const uint64_t* pmax) {
-  if (!jobs || njobs < 1 || !pmin || !pmax) return fail(RS_E_ARG, "rs_sieve_batch: bad args");
+  if (!jobs || njobs < 1 || njobs > RS_MAX_BATCH || !pmin || !pmax)
+    return fail(RS_E_ARG, "rs_sieve_batch: need 1..%d jobs", RS_MAX_BATCH);
+  rs_ctx* ctx = jobs[0] ? jobs[0]->ctx : nullptr;
+  if (!ctx) return fail(RS_E_ARG, "rs_sieve_batch: null job");
+  uint32_t T = jobs[0]->h.threads, hsize = 0, gen_max = 0, cap_max = 0;
+  uint64_t R = 0;
+  std::vector<uint64_t> ranges(njobs);
+  std::vector<uint32_t> gbs(njobs);
   for (int i = 0; i < njobs; i++) {
-    int rc = rs_sieve(jobs[i], pmin[i], pmax[i]);
-    if (rc) return rc;
+    rs_job* j = jobs[i];
+    if (!j || j->ctx != ctx) return fail(RS_E_ARG, "rs_sieve_batch: jobs must share one context");
+    if (pmin[i] < 3 || pmax[i] < pmin[i]) return fail(RS_E_ARG, "rs_sieve_batch: bad range of job %d", i);
+    uint64_t range;
+    uint32_t gb;
+    segment_shape(j, pmin[i], &range, &gb);
+    ranges[i] = range;
+    gbs[i] = gb;
+    hsize = std::max(hsize, j->h.hsize);
+    gen_max = std::max(gen_max, gb);
+    cap_max = std::max(cap_max, j->seg_cap);
+    R = std::max(R, rounds_for(pmin[i], pmax[i], range));
   }
+  if (bsgs_lds_bytes(hsize) + 64 > ctx->lds_per_block) return fail(RS_E_ARG, "rs_sieve_batch: table too large");
+  int bpc = 0;
+  if (bsgs_blocks_per_cu(T, hsize, &bpc) != hipSuccess || bpc < 1) bpc = 1;
+  const uint32_t bsgs_blocks = (uint32_t)bpc * ctx->cus;
+  const uint32_t setup_blocks = (cap_max + 3) / 4;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  // the pointers of every job (device to device), then its cursor and shape
+  for (int i = 0; i < njobs; i++) {
+    HIPCHK(hipMemcpyAsync(ctx->d_bjobs + i, jobs[i]->d_sj, sizeof(SegJob), hipMemcpyDeviceToDevice, s));
+    HIPCHK(launch_set_cursor(ctx->d_bjobs + i, pmin[i], pmax[i], T, ranges[i], gbs[i], s));
+  }
+  // one captured round per batch shape, replayed R times
+  const GraphKey key{(uint32_t)njobs, gen_max, setup_blocks, T, hsize, bsgs_blocks};
+  hipGraphExec_t exec = nullptr;
+  for (auto& g : ctx->graphs)
+    if (g.first == key) exec = g.second;
+  if (!exec) {
+    hipGraph_t graph;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t e = launch_round(ctx->d_bjobs, ctx->d_bstate, (uint32_t)njobs, gen_max, setup_blocks, T, hsize,
+                                bsgs_blocks, nullptr, nullptr, s);
+    hipError_t e2 = hipStreamEndCapture(s, &graph);
+    if (e != hipSuccess || e2 != hipSuccess)
+      return fail(RS_E_HIP, "rs_sieve_batch: capture failed: %s", hipGetErrorString(e ? e : e2));
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(RS_E_HIP, "rs_sieve_batch: instantiate failed: %s", hipGetErrorString(e));
+    ctx->graphs.push_back({key, exec});
+  }
+  rs_job* j0 = jobs[0];
+  if (!j0->timer_started) {
+    HIPCHK(hipEventRecord(j0->ev_all[0], s));
+    j0->timer_started = true;
+  }
+  for (uint64_t r = 0; r < R; r++) {
+    hipEvent_t e0, e1;
+    int rc = record_event(j0, &e0);
+    if (!rc) rc = record_event(j0, &e1);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(e0, s));
+    HIPCHK(hipGraphLaunch(exec, s));
+    HIPCHK(hipEventRecord(e1, s));
+  }
+  HIPCHK(hipEventRecord(j0->ev_all[1], s));
   return RS_OK;
 }
 
@@ -424,9 +526,17 @@ extern "C" int rs_job_kernel_time(rs_job* j, double* bsgs_s, uint64_t* launches,
   if (launches) *launches = j->ev_used / 2;
   if (total_s) {
     float ms = 0;
-    if (j->ev_used) HIPCHK(hipEventElapsedTime(&ms, j->ev_all[0], j->ev_all[1]));
+    if (j->timer_started) HIPCHK(hipEventElapsedTime(&ms, j->ev_all[0], j->ev_all[1]));
     *total_s = ms * 1e-3;
   }
+  return RS_OK;
+}
+
+extern "C" int rs_job_timer_reset(rs_job* j) {
+  if (!j) return fail(RS_E_ARG, "rs_job_timer_reset: null job");
+  HIPCHK(hipStreamSynchronize(j->ctx->stream));
+  j->ev_used = 0;
+  j->timer_started = false;
   return RS_OK;
 }
 
@@ -484,57 +594,55 @@ extern "C" int rs_primegen(rs_ctx* ctx, uint64_t pmin, uint64_t pmax, uint64_t* 
   if (pmin < 3 || pmax < pmin) return fail(RS_E_ARG, "rs_primegen: need 3 <= pmin <= pmax");
   HIPCHK(hipSetDevice(ctx->device));
   const uint32_t segcap = 1u << 20;
-  uint64_t *dp = nullptr;
-  SegState* ds = nullptr;
-  HIPCHK(hipMalloc((void**)&dp, (size_t)segcap * 8));
-  if (hipMalloc((void**)&ds, sizeof(SegState)) != hipSuccess) {
+  uint64_t* dp = nullptr;
+  SegJob* dj = nullptr;
+  BatchState* dbs = nullptr;
+  if (hipMalloc((void**)&dp, (size_t)segcap * 8) != hipSuccess || hipMalloc((void**)&dj, sizeof(SegJob)) != hipSuccess ||
+      hipMalloc((void**)&dbs, sizeof(BatchState)) != hipSuccess) {
     (void)hipFree(dp);
+    (void)hipFree(dj);
     return fail(RS_E_NOMEM, "rs_primegen: allocation failed");
   }
-  const uint64_t range = (uint64_t)(segcap - 16) * 30 / 8;  // cannot overflow the buffer
-  uint64_t total = 0;
+  // a segment can hold at most 8/30 of its integers (+ 16 at the edges)
+  const uint64_t range = (uint64_t)(segcap - 64) * 30 / 8;
+  const uint32_t gen_blocks = (uint32_t)(((range / 60 + 1) + 255) / 256);
+  SegJob h;
+  memset(&h, 0, sizeof(h));
+  h.primes = dp;
+  h.cap = segcap;
+  hipStream_t s = ctx->stream;
   int rc = RS_OK;
+  if (hipMemcpy(dj, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess ||
+      launch_set_cursor(dj, pmin, pmax, 256, range, gen_blocks, s) != hipSuccess)
+    rc = fail(RS_E_HIP, "rs_primegen: launch failed");
+  uint64_t total = 0;
   std::vector<uint64_t> tmp;
-  for (uint64_t p = pmin; p < pmax && rc == RS_OK;) {
-    uint64_t stop = p + range;
-    if (stop > pmax || stop < p) stop = pmax;
-    hipStream_t s = ctx->stream;
-    uint64_t pp = p;
-    if (launch_seg_begin(ds, s) != hipSuccess) rc = fail(RS_E_HIP, "rs_primegen: launch failed");
-    if (rc == RS_OK && pp < 114) {
-      const uint64_t stop_sm = stop > 114 ? 114 : stop;
-      if (launch_smallprimes(pp, stop_sm, dp, segcap, ds, s) != hipSuccess) rc = fail(RS_E_HIP, "launch failed");
-      pp = stop_sm;
+  const uint64_t R = rounds_for(pmin, pmax, range);
+  for (uint64_t r = 0; r < R && rc == RS_OK; r++) {
+    SegJob hs;
+    if (launch_primegen_round(dj, dbs, gen_blocks, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess ||
+        hipMemcpy(&hs, dj, sizeof(hs), hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = fail(RS_E_HIP, "rs_primegen: round failed");
+      break;
     }
-    if (rc == RS_OK && pp < stop) {
-      uint64_t ks = pp;
-      int32_t widx = 0;
-      rs_wheel_offset(&ks, &widx);
-      if (ks < stop && ks >= pp && launch_primegen(ks, stop, widx, dp, segcap, ds, s) != hipSuccess)
-        rc = fail(RS_E_HIP, "rs_primegen: launch failed");
+    const uint64_t n = hs.nprimes;
+    if (n > segcap) {
+      rc = fail(RS_E_OVERFLOW, "rs_primegen: segment overflow");
+      break;
     }
-    SegState hs;
-    if (rc == RS_OK && (hipStreamSynchronize(s) != hipSuccess ||
-                        hipMemcpy(&hs, ds, sizeof(hs), hipMemcpyDeviceToHost) != hipSuccess))
-      rc = fail(RS_E_HIP, "rs_primegen: sync failed");
-    if (rc == RS_OK) {
-      const uint64_t n = hs.nprimes;
-      if (n > segcap) {
-        rc = fail(RS_E_OVERFLOW, "rs_primegen: segment overflow");
-        break;
-      }
-      tmp.resize(n);
-      if (n && hipMemcpy(tmp.data(), dp, n * 8, hipMemcpyDeviceToHost) != hipSuccess)
-        rc = fail(RS_E_HIP, "rs_primegen: copy failed");
-      for (uint64_t i = 0; i < n && rc == RS_OK; i++) {
-        if (total < cap) out[total] = tmp[i];
-        total++;
-      }
+    tmp.resize(n);
+    if (n && hipMemcpy(tmp.data(), dp, n * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = fail(RS_E_HIP, "rs_primegen: copy failed");
+      break;
     }
-    p = stop;
+    for (uint64_t i = 0; i < n; i++) {
+      if (total < cap) out[total] = tmp[i];
+      total++;
+    }
   }
   (void)hipFree(dp);
-  (void)hipFree(ds);
+  (void)hipFree(dj);
+  (void)hipFree(dbs);
   *count = total;
   return rc;
 }

@@ -19,6 +19,7 @@ struct JobDev {
   // BSGS shape
   uint32_t algo;              // 0: three lists (reference shape), 1: one merged table
   uint32_t Q, M, QQ, MM;      // baby steps / giant steps, full and parity lists
+  uint32_t Qref;              // the reference's Q for this n-range (the setup invertibility test)
   uint32_t hsize, hmask;      // LDS hash slots (power of 2)
   uint32_t threads;           // BSGS workgroup size
   uint32_t record_factors;    // 0: claims clear bitmap bits only
@@ -67,11 +68,35 @@ struct Counters {
   uint64_t seg_overflow;      // segments whose primes exceeded the buffer
 };
 
-// Per-segment scheduling state (reset by a kernel at the start of a segment).
-struct SegState {
-  uint32_t nprimes;           // primes written by the generator
-  uint32_t work_next;         // BSGS work-queue head
-  uint32_t pad[2];
+// One job of a batch, device-resident.  The segment cursor advances on the
+// device (k_round_begin), so a round of kernels needs no host input and can be
+// replayed from a HIP graph.
+struct SegJob {
+  const JobDev* job;
+  PrimeRec* recs;
+  KRec* krecs;
+  uint64_t* primes;
+  uint32_t* bitmap;
+  FactorRec* fac;
+  Counters* cnt;
+  uint64_t p_cur, p_end;      // next segment start, end of the requested range
+  uint64_t range;             // integers per segment
+  uint64_t low, high;         // this round's generator range (after small primes)
+  uint32_t cap;               // prime capacity of one segment
+  uint32_t nprimes;           // primes written this round
+  int32_t widx;               // wheel index of 'low'
+  uint32_t threads;           // BSGS workgroup size of the launch (sets the strides)
+  uint32_t gen_blocks;        // primegen blocks this job needs per round
+  uint32_t pad;
+};
+
+#define RS_MAX_BATCH 256
+
+// Work queue of one round over all jobs of a batch.
+struct BatchState {
+  uint32_t njobs;
+  uint32_t work_next;
+  uint32_t item_base[RS_MAX_BATCH + 1];  // prefix sums of work items per job
 };
 
 }  // namespace rs

@@ -1,18 +1,25 @@
 // rs_kernels.hip -- HIP kernels for the k*b^n+-1 BSGS sieve on gfx950.
 //
-// One p-segment runs as: seg_begin -> primegen -> setup -> bsgs, all enqueued
-// on one stream with no host round trip (sizes are read on the device).
+// Work is organised in rounds over a batch of jobs (one job is a batch of
+// one).  A round is five launches on one stream that need no host input:
 //
-//  primegen  all base-2 strong probable primes in [low, high) with no prime
-//            factor <= 113 (the reference's getsegprimes.cl:128-182 output set)
-//  setup     one wavefront per prime: Montgomery constants, batch inversion of
-//            the k by wavefront product scans, power-residue / quadratic-
-//            character prefilter (setup.cl:11-209) and per-class compaction
-//            (the reference's sort.cl, folded in)
-//  bsgs      persistent workgroups pull (prime, class) items from a device
-//            work queue; the baby-step table lives entirely in LDS (32-bit
-//            fingerprint + 16-bit index per slot, no global spill), giant
-//            steps probe it (giant.cl:77-323)
+//  round_begin  advance each job's device-side segment cursor, emit the
+//               primes <= 113 (addsmallprimes.cl:8-27), find the wheel offset
+//               (cl_sieve.cpp:453-512)
+//  primegen     all base-2 strong probable primes of each job's segment with
+//               no prime factor <= 113 (getsegprimes.cl:128-182)
+//  setup        one wavefront per prime: Montgomery constants, batch inversion
+//               of the k by wavefront product scans, the power-residue /
+//               quadratic-character prefilter (setup.cl:11-209) and per-class
+//               compaction (the reference's sort.cl, folded in)
+//  round_items  prefix sums of the work items of every job
+//  bsgs         persistent workgroups pull (job, prime, class) items from one
+//               device queue; the baby-step table lives entirely in LDS
+//               (32-bit fingerprint + 16-bit index per slot, no global spill)
+//               and giant steps probe it (giant.cl:77-323)
+//
+// Because the cursors live on the device, the round can be captured once in
+// a HIP graph and replayed until every job's range is done.
 #include "rs_internal.h"
 #include "rs_mont.h"
 
@@ -73,11 +80,78 @@ __device__ bool sprp2(uint64_t N) {
   return false;
 }
 
-__global__ void k_seg_begin(SegState* seg) {
-  if (threadIdx.x == 0) {
-    seg->nprimes = 0;
-    seg->work_next = 0;
+// Start a job's range [pmin, pmax) with a given BSGS workgroup size.
+__global__ void k_set_cursor(SegJob* __restrict__ sj, uint64_t pmin, uint64_t pmax, uint32_t threads, uint64_t range,
+                             uint32_t gen_blocks) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sj->p_cur = pmin;
+    sj->p_end = pmax;
+    sj->threads = threads;
+    sj->range = range;
+    sj->gen_blocks = gen_blocks;
+    sj->nprimes = 0;
+    sj->low = sj->high = 0;
   }
+}
+
+// First N >= p coprime to 30 and its index in the wheel that starts at 7
+// (cl_sieve.cpp:453-512); returns false if N would pass 2^64.
+__device__ bool wheel_start(uint64_t p, uint64_t* N, int32_t* idx) {
+  // wheel position i starts at residue 7, 11, 13, 17, 19, 23, 29, 1 (mod 30)
+  const int8_t pos_of[30] = {-1, 7, -1, -1, -1, -1, -1, 0, -1, -1, -1, 1, -1, 2, -1,
+                             -1, -1, 3, -1, 4, -1, -1, -1, 5, -1, -1, -1, -1, -1, 6};
+  uint64_t n = p;
+  for (int i = 0; i < 7; i++) {
+    const int k = pos_of[n % 30];
+    if (k >= 0) {
+      *N = n;
+      *idx = k;
+      return true;
+    }
+    if (n + 1 < n) return false;
+    n++;
+  }
+  return false;
+}
+
+// One thread per job: next segment of each job (cl_sieve.cpp:1291-1320).
+__global__ void k_round_begin(SegJob* __restrict__ jobs, BatchState* __restrict__ bs, uint32_t njobs) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j == 0) {
+    bs->work_next = 0;
+    bs->njobs = njobs;
+  }
+  if (j >= njobs) return;
+  SegJob& J = jobs[j];
+  J.nprimes = 0;
+  J.low = J.high = 0;
+  if (J.p_cur >= J.p_end) return;
+  uint64_t p = J.p_cur;
+  uint64_t stop = p + J.range;
+  if (stop > J.p_end || stop < p) stop = J.p_end;
+  if (p < 114) {
+    const uint32_t small[30] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47,
+                                53, 59, 61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113};
+    const uint64_t stop_sm = stop > 114 ? 114 : stop;
+    uint32_t n = 0;
+    for (int i = 0; i < 30; i++)
+      if (small[i] >= p && small[i] < stop_sm) {
+        if (n < J.cap) J.primes[n] = small[i];
+        n++;
+      }
+    J.nprimes = n;
+    p = stop_sm;
+  }
+  if (p < stop) {
+    uint64_t N;
+    int32_t widx;
+    if (wheel_start(p, &N, &widx) && N < stop) {
+      J.low = N;
+      J.high = stop;
+      J.widx = widx;
+    }
+  }
+  J.p_cur = stop;
 }
 
 // ---------------------------------------------------------------------------
@@ -86,9 +160,13 @@ __global__ void k_seg_begin(SegState* seg) {
 // every lane of the workgroup tests them; passing primes are appended with one
 // global atomic per wavefront.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_primegen(uint64_t low, uint64_t high, int32_t widx,
-                                                  uint64_t* __restrict__ out, uint32_t cap,
-                                                  SegState* __restrict__ seg) {
+__global__ __launch_bounds__(256) void k_primegen(SegJob* __restrict__ jobs) {
+  SegJob& J = jobs[blockIdx.y];
+  const uint64_t low = J.low, high = J.high;
+  if (blockIdx.x >= J.gen_blocks || high <= low) return;
+  const int32_t widx = J.widx;
+  uint64_t* __restrict__ out = J.primes;
+  const uint32_t cap = J.cap;
   __shared__ uint64_t cand[256 * 16];
   __shared__ uint32_t ncand;
   const uint32_t tid = threadIdx.x;
@@ -131,7 +209,7 @@ __global__ __launch_bounds__(256) void k_primegen(uint64_t low, uint64_t high, i
     if (m) {
       const uint32_t leader = __ffsll((long long)m) - 1;
       uint32_t wbase = 0;
-      if (lane == leader) wbase = atomicAdd(&seg->nprimes, (uint32_t)__popcll(m));
+      if (lane == leader) wbase = atomicAdd(&J.nprimes, (uint32_t)__popcll(m));
       wbase = __shfl(wbase, leader);
       if (ok) {
         const uint32_t w = wbase + lane_rank(m);
@@ -139,19 +217,6 @@ __global__ __launch_bounds__(256) void k_primegen(uint64_t low, uint64_t high, i
       }
     }
   }
-}
-
-// primes <= 113 in [low, high) (addsmallprimes.cl:8-27)
-__global__ void k_smallprimes(uint64_t low, uint64_t high, uint64_t* __restrict__ out, uint32_t cap,
-                              SegState* __restrict__ seg) {
-  const uint32_t small[30] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47,
-                              53, 59, 61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113};
-  const uint32_t i = threadIdx.x;
-  if (i >= 30) return;
-  const uint64_t p = small[i];
-  if (p < low || p >= high) return;
-  const uint32_t w = atomicAdd(&seg->nprimes, 1u);
-  if (w < cap) out[w] = p;
 }
 
 // ---------------------------------------------------------------------------
@@ -183,12 +248,17 @@ __device__ __forceinline__ void powmod_lg2(uint64_t x0, uint64_t x1, uint64_t ex
 // d of the power-residue tests (setup.cl:22-48)
 __constant__ uint32_t g_prd[12] = {3, 4, 5, 7, 8, 9, 11, 13, 17, 19, 23, 29};
 
-__global__ __launch_bounds__(256) void k_setup(const JobDev* __restrict__ job, const uint64_t* __restrict__ primes,
-                                               const SegState* __restrict__ seg, PrimeRec* __restrict__ recs,
-                                               KRec* __restrict__ krecs, Counters* __restrict__ cnt, uint32_t cap) {
+__global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
+  const SegJob& J = jobs[blockIdx.y];
+  const JobDev* __restrict__ job = J.job;
+  const uint64_t* __restrict__ primes = J.primes;
+  PrimeRec* __restrict__ recs = J.recs;
+  KRec* __restrict__ krecs = J.krecs;
+  Counters* __restrict__ cnt = J.cnt;
+  const uint32_t cap = J.cap;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t pi = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const uint32_t ngen = seg->nprimes;
+  const uint32_t ngen = J.nprimes;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     atomicAdd((unsigned long long*)&cnt->primecount, (unsigned long long)ngen);
     atomicMax((unsigned long long*)&cnt->max_seg_primes, (unsigned long long)ngen);
@@ -199,7 +269,7 @@ __global__ __launch_bounds__(256) void k_setup(const JobDev* __restrict__ job, c
 
   const uint32_t base = job->base;
   const int K = job->kcount;
-  const uint32_t T = job->threads;
+  const uint32_t T = J.threads;
   const uint64_t p = primes[pi];
   const uint64_t q = mont_inv(p);
   const uint64_t one = r_mod(p);
@@ -218,7 +288,11 @@ __global__ __launch_bounds__(256) void k_setup(const JobDev* __restrict__ job, c
   // Montgomery k for k = lane and k = lane + 64; 'one' pads the products.
   const uint64_t mk0 = (int)lane < K ? mmul(job->ak[lane], r2, p, q) : one;
   const uint64_t mk1 = (int)lane + 64 < K ? mmul(job->ak[lane + 64], r2, p, q) : one;
-  const uint64_t gQ = basepow_sm(mbase, job->Q, base, p, q, one);
+  // The invertibility test multiplies the k by b^Qref, Qref being the
+  // reference's own split for this n-range (setup.cl:137-162): for a composite
+  // 2-PRP the test's outcome depends on that exponent, so every BSGS shape
+  // uses the same one and skips the same numbers.
+  const uint64_t gQ = basepow_sm(mbase, job->Qref, base, p, q, one);
 
   // inclusive prefix (s) and suffix (u) products over the wavefront
   uint64_t s0 = mk0, s1 = mk1, u0 = mk0, u1 = mk1;
@@ -280,7 +354,7 @@ __global__ __launch_bounds__(256) void k_setup(const JobDev* __restrict__ job, c
     }
     return;
   }
-  const uint64_t gQ_inv = mmul(inv_total, AB, p, q);  // setup.cl:164-165
+  const uint64_t gQref_inv = mmul(inv_total, AB, p, q);  // b^-Qref, setup.cl:164-165
   const uint64_t inv_k = mmul(inv_total, gQ, p, q);   // (prod of all k)^-1
 
   // k^-1 and the target hadj for both slots (setup.cl:180-188)
@@ -315,52 +389,91 @@ __global__ __launch_bounds__(256) void k_setup(const JobDev* __restrict__ job, c
     }
   }
 
-  // compaction by class: full, even, odd (the reference's sort.cl:40-71)
-  uint32_t nk[3];
-  uint32_t off = 0;
-  KRec* kr = krecs + (size_t)pi * K;
-#pragma unroll
-  for (int c = 1; c <= 3; c++) {
-    const uint64_t m0 = __ballot(cls0 == c), m1 = __ballot(cls1 == c);
-    const uint32_t c0 = __popcll(m0), c1 = __popcll(m1);
-    if (cls0 == c) {
-      KRec e;
-      e.hadj = h0;
-      e.kidx = lane;
-      e.cls = c;
-      kr[off + lane_rank(m0)] = e;
-    }
-    if (cls1 == c) {
-      KRec e;
-      e.hadj = h1;
-      e.kidx = lane + 64;
-      e.cls = c;
-      kr[off + c0 + lane_rank(m1)] = e;
-    }
-    nk[c - 1] = c0 + c1;
-    off += c0 + c1;
-  }
-  if (off == 0) {  // every k impossible (setup.cl:205-208)
+  if (__ballot(cls0 != 0 || cls1 != 0) == 0) {  // every k impossible (setup.cl:205-208)
     if (lane == 0) {
       R->nk[0] = R->nk[1] = R->nk[2] = 0;
       atomicAdd((unsigned long long*)&cnt->skipped, 1ull);
     }
     return;
   }
+  const bool merged = job->algo == 1;
 
-  // Constants of the BSGS kernel, lane-parallel:
-  //  lane 0: b^NMIN; 1: (b^-Q)^T; 2: (b^-QQ)^T; 3: b^T; 4: b^(2T)
-  const uint64_t g1 = mmul(gQ_inv, gQ_inv, p, q);
-  uint64_t y = one, yb = one;
+  // Constants of the BSGS kernel, lane-parallel in two rounds:
+  //  lanes 0, 3, 4, 5: b^NMIN, b^T, b^(2T), b^(Qref-1)  ->  b^-1 = b^(Qref-1) * b^-Qref
+  //  lanes 0, 1, 2:    b^-Q, b^-QT, b^-2QT (giant step of the table span Q)
+  uint64_t y = one, yb = mbase;
   uint32_t ye = 0;
-  if (lane == 0) { yb = mbase; ye = job->nmin; }
-  else if (lane == 1) { yb = gQ_inv; ye = T; }
-  else if (lane == 2) { yb = g1; ye = T; }
-  else if (lane == 3) { yb = mbase; ye = T; }
-  else if (lane == 4) { yb = mbase; ye = 2 * T; }
-  if (lane < 5) y = powmod_sm(yb, ye, p, q, one);
-  const uint64_t bn = shfl64(y, 0), gT0 = shfl64(y, 1), gT1 = shfl64(y, 2);
-  const uint64_t bT0 = shfl64(y, 3), bT1 = shfl64(y, 4);
+  if (lane == 0) ye = job->nmin;
+  else if (lane == 3) ye = T;
+  else if (lane == 4) ye = 2 * T;
+  else if (lane == 5) ye = job->Qref - 1;
+  if (lane == 0 || (lane >= 3 && lane <= 5)) y = powmod_sm(yb, ye, p, q, one);
+  const uint64_t bn = shfl64(y, 0), bT0 = shfl64(y, 3), bT1 = shfl64(y, 4);
+  const uint64_t binv = mmul(shfl64(y, 5), gQref_inv, p, q);
+  ye = 0;
+  if (lane == 0) ye = job->Q;
+  else if (lane == 1) ye = job->Q * T;
+  else if (lane == 2) ye = 2 * job->Q * T;
+  y = one;
+  if (lane < 3) y = powmod_sm(binv, ye, p, q, one);
+  const uint64_t gQ_inv = shfl64(y, 0), gT0 = shfl64(y, 1), gT1 = shfl64(y, 2);
+  const uint64_t g1 = mmul(gQ_inv, gQ_inv, p, q);
+
+  uint32_t nk[3];
+  KRec* kr = krecs + (size_t)pi * (merged ? 2 * K : K);
+  if (!merged) {
+    // compaction by class: full, even, odd (the reference's sort.cl:40-71)
+    uint32_t off = 0;
+#pragma unroll
+    for (int c = 1; c <= 3; c++) {
+      const uint64_t m0 = __ballot(cls0 == c), m1 = __ballot(cls1 == c);
+      const uint32_t c0 = __popcll(m0), c1 = __popcll(m1);
+      if (cls0 == c) {
+        KRec e;
+        e.hadj = h0;
+        e.kidx = lane;
+        e.cls = c;
+        kr[off + lane_rank(m0)] = e;
+      }
+      if (cls1 == c) {
+        KRec e;
+        e.hadj = h1;
+        e.kidx = lane + 64;
+        e.cls = c;
+        kr[off + c0 + lane_rank(m1)] = e;
+      }
+      nk[c - 1] = c0 + c1;
+      off += c0 + c1;
+    }
+  } else {
+    // One list against a table of even exponent offsets: a target with
+    // shift d searches b^(n - d) = hadj * b^-d.  Full-range k get d = 0 and
+    // d = 1, even-only k d = 0, odd-only k d = 1; cls holds d.
+    uint32_t off = 0;
+#pragma unroll
+    for (int d = 0; d <= 1; d++) {
+      const bool s0 = cls0 == 1 || cls0 == 2 + d, s1 = cls1 == 1 || cls1 == 2 + d;
+      const uint64_t m0 = __ballot(s0), m1 = __ballot(s1);
+      const uint32_t c0 = __popcll(m0), c1 = __popcll(m1);
+      if (s0) {
+        KRec e;
+        e.hadj = d ? mmul(h0, binv, p, q) : h0;
+        e.kidx = lane;
+        e.cls = d;
+        kr[off + lane_rank(m0)] = e;
+      }
+      if (s1) {
+        KRec e;
+        e.hadj = d ? mmul(h1, binv, p, q) : h1;
+        e.kidx = lane + 64;
+        e.cls = d;
+        kr[off + c0 + lane_rank(m1)] = e;
+      }
+      off += c0 + c1;
+    }
+    nk[0] = off;
+    nk[1] = nk[2] = 0;
+  }
   if (lane == 0) {
     PrimeRec rec;
     rec.p = p;
@@ -372,12 +485,12 @@ __global__ __launch_bounds__(256) void k_setup(const JobDev* __restrict__ job, c
     rec.g[1] = g1;
     rec.gT[0] = gT0;
     rec.gT[1] = gT1;
-    rec.bT[0] = bT0;
+    rec.bT[0] = merged ? bT1 : bT0;
     rec.bT[1] = bT1;
     rec.nk[0] = nk[0];
     rec.nk[1] = nk[1];
     rec.nk[2] = nk[2];
-    rec.flags = 0;
+    rec.flags = merged ? 1u : 0u;
     rec.pad[0] = rec.pad[1] = rec.pad[2] = 0;
     *R = rec;
   }
@@ -386,11 +499,6 @@ __global__ __launch_bounds__(256) void k_setup(const JobDev* __restrict__ job, c
 // ---------------------------------------------------------------------------
 // bsgs (three lists, the reference's giantfull/giantparity shape)
 // ---------------------------------------------------------------------------
-struct Tally {
-  uint32_t dups, claims;
-  uint64_t work, probes;
-};
-
 __device__ __forceinline__ void claim(const JobDev* __restrict__ job, uint32_t* __restrict__ bitmap,
                                       FactorRec* __restrict__ fac, Counters* __restrict__ cnt, uint64_t p,
                                       uint32_t kidx, uint32_t n, uint32_t* s_dups, uint32_t* s_claims) {
@@ -456,52 +564,62 @@ __device__ __forceinline__ int tab_lookup(const uint32_t* fp, const uint16_t* ix
   }
 }
 
+// One persistent kernel serves every job of a batch.  An item is (job, prime,
+// class) for a job with the reference's three tables (giant.cl giantfull /
+// giantparity) and (job, prime) for a job with the merged table of
+// b^(NMIN + 2j), whose targets carry their shift d.  The LDS holds the largest
+// table of the batch; each item uses its own job's table size.
 template <int T>
-__global__ __launch_bounds__(T) void k_bsgs3(const JobDev* __restrict__ job, const PrimeRec* __restrict__ recs,
-                                             const KRec* __restrict__ krecs, SegState* __restrict__ seg,
-                                             uint32_t* __restrict__ bitmap, FactorRec* __restrict__ fac,
-                                             Counters* __restrict__ cnt, uint32_t cap) {
+__global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchState* __restrict__ bs) {
   extern __shared__ uint32_t lds[];
-  const uint32_t hsize = job->hsize, hmask = job->hmask;
-  uint32_t* fp = lds;
-  uint16_t* ix = (uint16_t*)(lds + hsize);
-  __shared__ uint32_t s_item, s_order, s_dups, s_claims;
-  __shared__ unsigned long long s_work, s_probe;
+  __shared__ uint32_t s_item, s_job, s_order, s_dups, s_claims;
   const uint32_t tid = threadIdx.x;
-  if (tid == 0) {
-    s_dups = 0;
-    s_claims = 0;
-    s_work = 0;
-    s_probe = 0;
-  }
-  const uint32_t ngen = seg->nprimes;
-  const uint32_t np = ngen < cap ? ngen : cap;
-  const uint32_t nitems = np * 3;
-  const uint32_t base = job->base;
-  const int K = job->kcount;
-  const uint32_t nmin = job->nmin, nmax = job->nmax;
+  const uint32_t njobs = bs->njobs;
+  const uint32_t nitems = bs->item_base[njobs];
 
   for (;;) {
-    if (tid == 0) s_item = atomicAdd(&seg->work_next, 1u);
+    if (tid == 0) {
+      const uint32_t it = atomicAdd(&bs->work_next, 1u);
+      uint32_t j = 0;
+      if (it < nitems)
+        while (it >= bs->item_base[j + 1]) j++;
+      s_item = it;
+      s_job = j;
+      s_dups = 0;
+      s_claims = 0;
+    }
     __syncthreads();
-    const uint32_t item = s_item;
+    const uint32_t item = s_item, jj = s_job;
     __syncthreads();
     if (item >= nitems) break;
-    const uint32_t pi = item / 3, c = item % 3;
-    const PrimeRec* __restrict__ R = recs + pi;
+    const SegJob& J = jobs[jj];
+    const JobDev* __restrict__ job = J.job;
+    const bool merged = job->algo == 1;
+    const uint32_t local = item - bs->item_base[jj];
+    const uint32_t pi = merged ? local : local / 3, c = merged ? 0 : local % 3;
+    const PrimeRec* __restrict__ R = J.recs + pi;
     const uint32_t nk = R->nk[c];
     if (nk == 0) continue;
-    const uint32_t cp = c ? 1 : 0;
+    const uint32_t hmask = job->hmask, hsize = job->hsize;
+    uint32_t* fp = lds;
+    uint16_t* ix = (uint16_t*)(lds + hsize);
+    const uint32_t base = job->base;
+    const int K = job->kcount;
+    const uint32_t nmin = job->nmin, nmax = job->nmax;
+    const uint32_t cp = (c && !merged) ? 1 : 0;
     const uint64_t p = R->p, q = R->q, one = R->one, mbase = R->mbase;
     const uint32_t Ql = cp ? job->QQ : job->Q;
     const uint32_t Ml = cp ? job->MM : job->M;
-    const uint32_t stride = cp ? 2 : 1;
+    const uint32_t stride = (cp || merged) ? 2 : 1;
     const uint32_t nstart = nmin + (c == 2 ? 1 : 0);
     const uint32_t nent = Ql / stride;
     const uint64_t v0 = (c == 2) ? mul_base(R->bn, mbase, base, p, q) : R->bn;
     const uint64_t bstep = R->bT[cp];
     const uint32_t koff = (c >= 1 ? R->nk[0] : 0) + (c >= 2 ? R->nk[1] : 0);
-    const KRec* __restrict__ kr = krecs + (size_t)pi * K + koff;
+    const KRec* __restrict__ kr = J.krecs + (size_t)pi * (merged ? 2 * K : K) + koff;
+    uint32_t* __restrict__ bitmap = J.bitmap;
+    FactorRec* __restrict__ fac = J.fac;
+    Counters* __restrict__ cnt = J.cnt;
 
     for (uint32_t s = tid; s < hsize; s += T) fp[s] = 0u;
     if (tid == 0) s_order = 0xFFFFFFFFu;
@@ -519,8 +637,9 @@ __global__ __launch_bounds__(T) void k_bsgs3(const JobDev* __restrict__ job, con
     }
     __syncthreads();
     const uint32_t order = s_order;
+    unsigned long long work, probes;
     if (order != 0xFFFFFFFFu) {
-      // ord_p(b) (or its even multiple) is below the table size: keep only
+      // ord_p(b) (or its even multiple) is below the table span: keep only
       // j < order (giant.cl:121-134), then solve without giant steps.
       __syncthreads();
       for (uint32_t s = tid; s < hsize; s += T) fp[s] = 0u;
@@ -532,19 +651,17 @@ __global__ __launch_bounds__(T) void k_bsgs3(const JobDev* __restrict__ job, con
         v = mmul(v, bstep, p, q);
       }
       __syncthreads();
-      if (tid < nk) {
-        const int j = tab_lookup(fp, ix, hmask, kr[tid].hadj, mbase, base, nstart, p, q, one);
+      for (uint32_t i = tid; i < nk; i += T) {
+        const int j = tab_lookup(fp, ix, hmask, kr[i].hadj, mbase, base, nstart, p, q, one);
         if (j >= 0) {
-          const uint32_t kidx = kr[tid].kidx;
-          for (uint64_t n = (uint64_t)nstart + (uint32_t)j; n <= nmax; n += order)
+          const uint32_t kidx = kr[i].kidx;
+          const uint32_t d = merged ? kr[i].cls : 0;
+          for (uint64_t n = (uint64_t)nstart + (uint32_t)j + d; n <= nmax; n += order)
             claim(job, bitmap, fac, cnt, p, kidx, (uint32_t)n, &s_dups, &s_claims);
         }
       }
-      if (tid == 0) {
-        atomicAdd(&s_work, (unsigned long long)(order / stride + nk));
-        atomicAdd(&s_probe, (unsigned long long)nk);
-        atomicAdd((unsigned long long*)&cnt->order_lists, 1ull);
-      }
+      work = order / stride + nk;
+      probes = nk;
     } else {
       // giant steps: thread tid handles q = tid, tid + T, ...
       const uint64_t g = R->g[cp], gT = R->gT[cp];
@@ -554,92 +671,94 @@ __global__ __launch_bounds__(T) void k_bsgs3(const JobDev* __restrict__ job, con
         for (uint32_t i = 0; i < nk; i++) {
           const uint64_t gamma = mmul(kr[i].hadj, Gq, p, q);
           const int j = tab_lookup(fp, ix, hmask, gamma, mbase, base, nstart, p, q, one);
-          if (j >= 0) claim(job, bitmap, fac, cnt, p, kr[i].kidx, nq + (uint32_t)j, &s_dups, &s_claims);
+          if (j >= 0)
+            claim(job, bitmap, fac, cnt, p, kr[i].kidx, nq + (uint32_t)j + (merged ? kr[i].cls : 0u), &s_dups,
+                  &s_claims);
         }
         Gq = mmul(Gq, gT, p, q);
       }
-      if (tid == 0) {
-        atomicAdd(&s_work, (unsigned long long)nent + (unsigned long long)Ml * (nk + 1));
-        atomicAdd(&s_probe, (unsigned long long)Ml * nk);
-      }
+      work = (unsigned long long)nent + (unsigned long long)Ml * (nk + 1);
+      probes = (unsigned long long)Ml * nk;
     }
-    if (tid == 0) atomicAdd((unsigned long long*)&cnt->lists[c], 1ull);
+    __syncthreads();
+    if (tid == 0) {
+      if (s_dups) atomicAdd((unsigned long long*)&cnt->dups, (unsigned long long)s_dups);
+      if (s_claims) atomicAdd((unsigned long long*)&cnt->claims, (unsigned long long)s_claims);
+      atomicAdd((unsigned long long*)&cnt->work_mmul, work);
+      atomicAdd((unsigned long long*)&cnt->work_probe, probes);
+      atomicAdd((unsigned long long*)&cnt->lists[c], 1ull);
+      if (order != 0xFFFFFFFFu) atomicAdd((unsigned long long*)&cnt->order_lists, 1ull);
+    }
     __syncthreads();
   }
-  __syncthreads();
-  if (tid == 0) {
-    if (s_dups) atomicAdd((unsigned long long*)&cnt->dups, (unsigned long long)s_dups);
-    if (s_claims) atomicAdd((unsigned long long*)&cnt->claims, (unsigned long long)s_claims);
-    atomicAdd((unsigned long long*)&cnt->work_mmul, s_work);
-    atomicAdd((unsigned long long*)&cnt->work_probe, s_probe);
+}
+
+// Work items of every job in prefix order; one thread (a handful of jobs).
+__global__ void k_round_items(SegJob* __restrict__ jobs, BatchState* __restrict__ bs, uint32_t njobs) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t base = 0;
+  for (uint32_t j = 0; j < njobs; j++) {
+    bs->item_base[j] = base;
+    const uint32_t np = jobs[j].nprimes < jobs[j].cap ? jobs[j].nprimes : jobs[j].cap;
+    base += np * (jobs[j].job->algo == 1 ? 1u : 3u);
   }
+  bs->item_base[njobs] = base;
 }
 
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_seg_begin(SegState* seg, hipStream_t s) {
-  hipLaunchKernelGGL(k_seg_begin, dim3(1), dim3(64), 0, s, seg);
-  return hipGetLastError();
-}
-
-hipError_t launch_smallprimes(uint64_t low, uint64_t high, uint64_t* out, uint32_t cap, SegState* seg,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(k_smallprimes, dim3(1), dim3(64), 0, s, low, high, out, cap, seg);
-  return hipGetLastError();
-}
-
-hipError_t launch_primegen(uint64_t low, uint64_t high, int32_t widx, uint64_t* out, uint32_t cap, SegState* seg,
-                           hipStream_t s) {
-  if (high <= low) return hipSuccess;
-  const uint64_t nthreads = (high - low) / 60 + 1;
-  const uint64_t nblocks = (nthreads + 255) / 256;
-  if (nblocks > 0x7fffffffull) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_primegen, dim3((uint32_t)nblocks), dim3(256), 0, s, low, high, widx, out, cap, seg);
-  return hipGetLastError();
-}
-
-hipError_t launch_setup(const JobDev* job, const uint64_t* primes, const SegState* seg, PrimeRec* recs, KRec* krecs,
-                        Counters* cnt, uint32_t cap, hipStream_t s) {
-  const uint32_t nblocks = (cap + 3) / 4;
-  hipLaunchKernelGGL(k_setup, dim3(nblocks), dim3(256), 0, s, job, primes, seg, recs, krecs, cnt, cap);
-  return hipGetLastError();
-}
-
 size_t bsgs_lds_bytes(uint32_t hsize) { return (size_t)hsize * 6; }
 
 hipError_t bsgs_blocks_per_cu(uint32_t threads, uint32_t hsize, int* out) {
   const size_t lds = bsgs_lds_bytes(hsize);
   switch (threads) {
-    case 128: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_bsgs3<128>, (int)threads, lds);
-    case 256: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_bsgs3<256>, (int)threads, lds);
-    case 512: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_bsgs3<512>, (int)threads, lds);
-    case 1024: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_bsgs3<1024>, (int)threads, lds);
+    case 128: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_bsgs<128>, 128, lds);
+    case 256: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_bsgs<256>, 256, lds);
+    case 512: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_bsgs<512>, 512, lds);
+    case 1024: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_bsgs<1024>, 1024, lds);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_bsgs(uint32_t threads, uint32_t hsize, uint32_t nblocks, const JobDev* job, const PrimeRec* recs,
-                       const KRec* krecs, SegState* seg, uint32_t* bitmap, FactorRec* fac, Counters* cnt,
-                       uint32_t cap, hipStream_t s) {
+hipError_t launch_set_cursor(SegJob* sj, uint64_t pmin, uint64_t pmax, uint32_t threads, uint64_t range,
+                             uint32_t gen_blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_cursor, dim3(1), dim3(64), 0, s, sj, pmin, pmax, threads, range, gen_blocks);
+  return hipGetLastError();
+}
+
+// One round of a batch: the five launches of the header comment.
+hipError_t launch_round(SegJob* jobs, BatchState* bs, uint32_t njobs, uint32_t gen_blocks, uint32_t setup_blocks,
+                        uint32_t threads, uint32_t hsize, uint32_t bsgs_blocks, hipEvent_t ev0, hipEvent_t ev1,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_round_begin, dim3((njobs + 63) / 64), dim3(64), 0, s, jobs, bs, njobs);
+  if (gen_blocks) hipLaunchKernelGGL(k_primegen, dim3(gen_blocks, njobs), dim3(256), 0, s, jobs);
+  hipLaunchKernelGGL(k_setup, dim3(setup_blocks, njobs), dim3(256), 0, s, jobs);
+  hipLaunchKernelGGL(k_round_items, dim3(1), dim3(64), 0, s, jobs, bs, njobs);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (ev0) {
+    e = hipEventRecord(ev0, s);
+    if (e != hipSuccess) return e;
+  }
   const size_t lds = bsgs_lds_bytes(hsize);
   switch (threads) {
-    case 128:
-      hipLaunchKernelGGL(k_bsgs3<128>, dim3(nblocks), dim3(128), lds, s, job, recs, krecs, seg, bitmap, fac, cnt, cap);
-      break;
-    case 256:
-      hipLaunchKernelGGL(k_bsgs3<256>, dim3(nblocks), dim3(256), lds, s, job, recs, krecs, seg, bitmap, fac, cnt, cap);
-      break;
-    case 512:
-      hipLaunchKernelGGL(k_bsgs3<512>, dim3(nblocks), dim3(512), lds, s, job, recs, krecs, seg, bitmap, fac, cnt, cap);
-      break;
-    case 1024:
-      hipLaunchKernelGGL(k_bsgs3<1024>, dim3(nblocks), dim3(1024), lds, s, job, recs, krecs, seg, bitmap, fac, cnt,
-                         cap);
-      break;
-    default:
-      return hipErrorInvalidValue;
+    case 128: hipLaunchKernelGGL(k_bsgs<128>, dim3(bsgs_blocks), dim3(128), lds, s, jobs, bs); break;
+    case 256: hipLaunchKernelGGL(k_bsgs<256>, dim3(bsgs_blocks), dim3(256), lds, s, jobs, bs); break;
+    case 512: hipLaunchKernelGGL(k_bsgs<512>, dim3(bsgs_blocks), dim3(512), lds, s, jobs, bs); break;
+    case 1024: hipLaunchKernelGGL(k_bsgs<1024>, dim3(bsgs_blocks), dim3(1024), lds, s, jobs, bs); break;
+    default: return hipErrorInvalidValue;
   }
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (ev1) e = hipEventRecord(ev1, s);
+  return e;
+}
+
+// The generator alone (rs_primegen): round_begin + primegen for one job.
+hipError_t launch_primegen_round(SegJob* jobs, BatchState* bs, uint32_t gen_blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, s, jobs, bs, 1u);
+  if (gen_blocks) hipLaunchKernelGGL(k_primegen, dim3(gen_blocks, 1), dim3(256), 0, s, jobs);
   return hipGetLastError();
 }
 

@@ -1,0 +1,84 @@
+"""The rssieve command line on the GPU: ABCD in, factor file out, same
+terms eliminated as the oracle; checkpoint resume; self test."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import rssieve_amd as R
+from util import bitmap_terms, full_bitmap
+
+pytestmark = pytest.mark.gpu
+
+
+def _make_abcd(path, base, ks, cs, nmin, nmax, density, seed):
+    bm, wps = full_bitmap(len(ks), nmin, nmax, density=density, seed=seed)
+    R.abcd_write(str(path), base, ks, cs, nmin, nmax, bm, wps, sieved_to=3)
+    a = R.ABCD(str(path))
+    return a
+
+
+def _run(args, cwd, timeout=120):
+    print("rssieve", " ".join(args), flush=True)
+    return subprocess.run([R.CLI_PATH] + args, cwd=cwd, capture_output=True, text=True, timeout=timeout)
+
+
+def _factor_lines(path):
+    return [l for l in open(path).read().splitlines() if l.strip()]
+
+
+def test_cli_matches_oracle(tmp_path):
+    base, ks, cs = 2, [3, 5, 7, 11, 13, 15, 17, 19], [1, -1, 1, -1, 1, -1, 1, -1]
+    nmin, nmax = 20000, 60000
+    a = _make_abcd(tmp_path / "in.abcd", base, ks, cs, nmin, nmax, 0.6, 1)
+    r = _run(["-p", "1e8", "-P", "100500000", "-i", "in.abcd", "-f", "out.txt"], str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Sequences read: 8" in r.stdout and "Base = 2" in r.stdout
+    lines = _factor_lines(tmp_path / "out.txt")
+    job = O.make_job(base, [int(x) for x in a.k], [int(x) for x in a.c], a.nmin, a.nmax)
+    bm = a.bitmap()
+    cnt, facs, nf = O.sieve(job, 10**8, 100500000, bm.copy())
+    want = {(int(a.k[i]), int(a.c[i]), n) for (p, n, i) in facs}
+    got = set()
+    for l in lines:
+        p, k, s, n = R.parse_factor(l, 2)
+        c = 1 if s == "+" else -1
+        assert R.verify_factor(p, k, n, c, 2) == 1
+        got.add((k, c, n))
+    assert got == want
+    # lines sorted by (p, n) within the batch, reference format
+    ps = [int(l.split(" | ")[0]) for l in lines]
+    assert ps == sorted(ps)
+    summary = [l for l in r.stdout.splitlines() if l.startswith("factors ")][-1]
+    assert summary.startswith("factors %d, prime count %d, checksum " % (len(want), cnt.primecount))
+    assert summary.endswith("duplicate factors %d" % cnt.dups)
+
+
+def test_cli_no_factors_and_resume(tmp_path):
+    base, ks, cs = 2, [1001, 3003], [1, -1]
+    _make_abcd(tmp_path / "in.abcd", base, ks, cs, 1000000, 1001000, 0.5, 2)
+    r = _run(["-p", "1000000007", "-P", "1000000100", "-i", "in.abcd", "-f", "f.txt"], str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert open(tmp_path / "f.txt").read() == "no factors\n"
+    # a completed run leaves a checkpoint at p == P: running again reports completion
+    r2 = _run(["-p", "1000000007", "-P", "1000000100", "-i", "in.abcd", "-f", "f.txt"], str(tmp_path))
+    assert r2.returncode == 0 and "Workunit complete." in r2.stdout
+
+
+def test_cli_errors(tmp_path):
+    (tmp_path / "bad.abcd").write_text("ABCD 3*4^$a+1 [10]\n")
+    r = _run(["-p", "1e6", "-P", "2e6", "-i", "bad.abcd"], str(tmp_path))
+    assert r.returncode != 0 and "perfect power" in r.stdout
+    (tmp_path / "ok.abcd").write_text("ABCD 3*2^$a+1 [10]\n")
+    r = _run(["-i", "ok.abcd"], str(tmp_path))
+    assert r.returncode != 0 and "-p and -P arguments are required" in r.stdout
+    r = _run(["-p", "1e6", "-P", "2e6"], str(tmp_path))
+    assert r.returncode != 0 and "no input file" in r.stdout
+
+
+def test_cli_selftest(tmp_path):
+    r = _run(["-s"], str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "1000133 | 91549*2^54378870+1 found" in r.stdout

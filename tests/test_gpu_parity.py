@@ -134,3 +134,76 @@ def test_sieve_k_above_2_31(ctx, k31):
     gpu = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, prm)
     orc = oracle_run(base, ks, cs, nmin, nmax, bm, pmin, pmax, k31=k31)
     _compare(base, ks, cs, nmin, nmax, wps, bm, gpu, orc, k31=k31)
+
+
+def merged_params(ctx, nmin, nmax, K, entries=4096, hsize=8192, threads=256):
+    """Parameters of the merged-table path: one table of 'entries' even
+    exponent offsets (span 2*entries)."""
+    prm = R.reference_params(nmin, nmax, K, 65536, ctx.compute_units)
+    prm.algo = R.ALGO_MERGED
+    prm.Q = 2 * entries
+    prm.hsize = hsize
+    prm.threads = threads
+    return prm
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("shape", [(1024, 2048, 128), (4096, 8192, 256), (8192, 16384, 512)],
+                         ids=["e1024", "e4096", "e8192"])
+def test_merged_table_same_eliminations(ctx, case, shape):
+    """The merged-table BSGS eliminates exactly the terms the reference-shaped
+    sieve (and the oracle) eliminates; its duplicate count follows its own
+    search windows, so it is not compared."""
+    name, base, K, nmin, nmax, pmin, pmax, seed, dens = case
+    print("merged case", case, shape, flush=True)
+    ks, cs = _draw(base, K, seed)
+    bm, wps = full_bitmap(K, nmin, nmax, density=dens, seed=seed)
+    prm = merged_params(ctx, nmin, nmax, K, *shape)
+    gcnt, gfac, gbm = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, prm)
+    ocnt, ofac, obm = oracle_run(base, ks, cs, nmin, nmax, bm, pmin, pmax)
+    assert gcnt["primecount"] == ocnt["primecount"]
+    assert gcnt["skipped"] == ocnt["skipped"]
+    assert gcnt["claims"] == ocnt["claims"]
+    assert np.array_equal(gbm, obm)
+    assert {(int(f["kidx"]), int(f["n"])) for f in gfac} == {(k, n) for (_, n, k) in ofac}
+    for f in gfac:
+        i = int(f["kidx"])
+        assert R.verify_factor(int(f["p"]), ks[i], int(f["n"]), cs[i], base) in (1, -1)
+
+
+def test_batch_of_uneven_jobs_matches_oracle(ctx):
+    """Several independent jobs (bases, sequence counts, n-spans, p-ranges and
+    both BSGS shapes) in one batch: one persistent BSGS launch per round,
+    replayed from a HIP graph.  Each job's result equals the oracle's."""
+    specs = [
+        # base, K, nmin, nmax, pmin, pmax, seed, algo, entries, hsize
+        (2, 10, 100000, 110000, 10**9, 10**9 + 10**6, 21, R.ALGO_THREE_LISTS, 0, 0),
+        (3, 1, 2000, 40000, 10**7, 10**7 + 3 * 10**5, 22, R.ALGO_MERGED, 2048, 8192),
+        (5, 7, 10000, 30000, 2**33, 2**33 + 2 * 10**5, 23, R.ALGO_THREE_LISTS, 0, 0),
+        (6, 3, 500, 2500, 300, 70000, 24, R.ALGO_MERGED, 1024, 4096),
+        (10, 25, 40000, 60000, 10**6, 10**6 + 4 * 10**5, 25, R.ALGO_MERGED, 4096, 16384),
+        (2, 100, 3000000, 3010000, 2**48, 2**48 + 2 * 10**5, 26, R.ALGO_THREE_LISTS, 0, 0),
+    ]
+    jobs, inputs = [], []
+    for (base, K, nmin, nmax, pmin, pmax, seed, algo, ent, hs) in specs:
+        ks, cs = _draw(base, K, seed)
+        bm, wps = full_bitmap(K, nmin, nmax, density=0.5, seed=seed)
+        prm = R.reference_params(nmin, nmax, K, 65536, ctx.compute_units)
+        if algo == R.ALGO_MERGED:
+            prm.algo, prm.Q, prm.hsize = algo, 2 * ent, hs
+        jobs.append(ctx.job(base, ks, cs, nmin, nmax, bm, wps, prm))
+        inputs.append((base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, algo))
+    print("batch of", len(jobs), flush=True)
+    R.sieve_batch(jobs, [x[7] for x in inputs], [x[8] for x in inputs])
+    for job, (base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, algo) in zip(jobs, inputs):
+        cnt = job.counters().as_dict()
+        facs = job.drain()
+        gbm = job.bitmap()
+        job.close()
+        ocnt, ofac, obm = oracle_run(base, ks, cs, nmin, nmax, bm, pmin, pmax)
+        assert cnt["primecount"] == ocnt["primecount"]
+        assert cnt["claims"] == ocnt["claims"]
+        if algo == R.ALGO_THREE_LISTS:
+            assert cnt["dups"] == ocnt["dups"]
+        assert np.array_equal(gbm, obm)
+        assert {(int(f["kidx"]), int(f["n"])) for f in facs} == {(k, n) for (_, n, k) in ofac}
