@@ -90,6 +90,9 @@ int rs_open(int device, rs_ctx **out);
 void rs_close(rs_ctx *ctx);
 /* Message of the last failure on this thread ("" if none). */
 const char *rs_last_error(void);
+/* The HIP stream (hipStream_t) the context enqueues on, for callers that
+ * order their own work with the sieve's. */
+int rs_context_stream(rs_ctx *ctx, void **stream);
 /* Device facts: compute units, LDS bytes per workgroup, device name. */
 int rs_device_info(rs_ctx *ctx, uint32_t *compute_units, uint32_t *lds_per_block, char *name, size_t name_len);
 

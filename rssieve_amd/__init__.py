@@ -128,6 +128,24 @@ _SIGS = {
                                   C.POINTER(C.c_char), C.POINTER(C.c_uint32)]),
     "rs_state_write": (C.c_int, [C.c_char_p, C.POINTER(State), C.POINTER(C.c_int)]),
     "rs_state_read": (C.c_int, [C.c_char_p, C.POINTER(State), C.POINTER(C.c_int)]),
+    "rs_context_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    # rssieve_mlp.h
+    "rs_tune_candidates": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(Params), C.c_int,
+                                     C.POINTER(C.c_int)]),
+    "rs_autotune": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(Params), C.c_int,
+                              C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.POINTER(C.c_int)]),
+    "rs_job_reconfigure": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
+    "rs_mlp_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "rs_mlp_destroy": (None, [C.c_void_p]),
+    "rs_mlp_num_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "rs_mlp_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "rs_mlp_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "rs_mlp_forward": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
+    "rs_mlp_loss_grad": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                   C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "rs_mlp_adam_step": (C.c_int, [C.c_void_p, C.c_float]),
+    "rs_mlp_train": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
+                               C.c_int, C.c_int, C.c_float, C.c_uint64, C.POINTER(C.c_float)]),
 }
 
 
@@ -292,6 +310,16 @@ class Context:
         _check(lib().rs_primegen(self._h, pmin, pmax, _u64p(out), cap, C.byref(cnt)))
         return out[:min(cnt.value, cap)]
 
+    def tune_candidates(self, nmin, nmax, kcount):
+        cap = 64
+        arr = (Params * cap)()
+        n = C.c_int()
+        _check(lib().rs_tune_candidates(self._h, nmin, nmax, kcount, arr, cap, C.byref(n)))
+        return [arr[i] for i in range(min(n.value, cap))]
+
+    def mlp(self, n_in, n_hidden, n_out, seed=1):
+        return MLP(self, n_in, n_hidden, n_out, seed)
+
     def job(self, base, k, c, nmin, nmax, bitmap, words_per_seq, params):
         return Job(self, base, k, c, nmin, nmax, bitmap, words_per_seq, params)
 
@@ -357,6 +385,19 @@ class Job:
         _check(lib().rs_job_kernel_time(self._h, C.byref(t), C.byref(n), C.byref(tot)))
         return t.value, n.value, tot.value
 
+    def reconfigure(self, params):
+        _check(lib().rs_job_reconfigure(self._h, C.byref(params)))
+        self.params = params
+
+    def autotune(self, pmin, width, candidates):
+        n = len(candidates)
+        arr = (Params * n)(*candidates)
+        secs = (C.c_double * n)()
+        pr = (C.c_uint64 * n)()
+        best = C.c_int()
+        _check(lib().rs_autotune(self._h, pmin, width, arr, n, secs, pr, C.byref(best)))
+        return best.value, list(secs), list(pr)
+
     def timer_reset(self):
         _check(lib().rs_job_timer_reset(self._h))
 
@@ -377,3 +418,89 @@ def sieve_batch(jobs, pmins, pmaxs):
     lo = (C.c_uint64 * len(jobs))(*pmins)
     hi = (C.c_uint64 * len(jobs))(*pmaxs)
     _check(lib().rs_sieve_batch(arr, len(jobs), lo, hi))
+
+
+def _f32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class MLP:
+    """The cost-model MLP (rssieve_mlp.h): n_in -> H -> H -> n_out, ReLU, fp32,
+    trained and evaluated by the library's HIP kernels."""
+
+    def __init__(self, ctx, n_in, n_hidden, n_out, seed=1):
+        h = C.c_void_p()
+        _check(lib().rs_mlp_create(ctx._h, n_in, n_hidden, n_out, seed, C.byref(h)))
+        self._h = h
+        self.n_in, self.H, self.n_out = n_in, n_hidden, n_out
+        n = C.c_uint64()
+        _check(lib().rs_mlp_num_params(h, C.byref(n)))
+        self.nparam = n.value
+
+    def shapes(self):
+        a, H, C_ = self.n_in, self.H, self.n_out
+        return [(a, H), (H,), (H, H), (H,), (H, C_), (C_,)]
+
+    def get_params(self):
+        out = np.zeros(self.nparam, np.float32)
+        _check(lib().rs_mlp_get_params(self._h, _f32p(out)))
+        res, o = [], 0
+        for sh in self.shapes():
+            k = int(np.prod(sh))
+            res.append(out[o:o + k].reshape(sh).copy())
+            o += k
+        return res
+
+    def set_params(self, params):
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(p, np.float32).reshape(-1) for p in params]))
+        assert flat.size == self.nparam
+        _check(lib().rs_mlp_set_params(self._h, _f32p(flat)))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], self.n_out), np.float32)
+        _check(lib().rs_mlp_forward(self._h, _f32p(X), X.shape[0], _f32p(Y)))
+        return Y
+
+    def loss_grad(self, X, T, M=None):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        Mp = None
+        if M is not None:
+            M = np.ascontiguousarray(M, np.float32)
+            Mp = _f32p(M)
+        g = np.zeros(self.nparam, np.float32)
+        loss = C.c_float()
+        _check(lib().rs_mlp_loss_grad(self._h, _f32p(X), _f32p(T), Mp, X.shape[0], C.byref(loss), _f32p(g)))
+        res, o = [], 0
+        for sh in self.shapes():
+            k = int(np.prod(sh))
+            res.append(g[o:o + k].reshape(sh).copy())
+            o += k
+        return loss.value, res
+
+    def adam_step(self, lr):
+        _check(lib().rs_mlp_adam_step(self._h, lr))
+
+    def train(self, X, T, M=None, epochs=50, batch=256, lr=1e-3, seed=1):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        Mp = None
+        if M is not None:
+            M = np.ascontiguousarray(M, np.float32)
+            Mp = _f32p(M)
+        loss = C.c_float()
+        _check(lib().rs_mlp_train(self._h, _f32p(X), _f32p(T), Mp, X.shape[0], epochs, batch, lr, seed,
+                                  C.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self._h:
+            lib().rs_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/rssieve.h"
+#include "../../include/rssieve_mlp.h"
 #include "rs_internal.h"
 
 namespace rs {
@@ -220,6 +221,47 @@ extern "C" void rs_job_destroy(rs_job* job) {
 
 static bool pow2(uint32_t x) { return x && !(x & (x - 1)); }
 
+// Checks of a BSGS configuration against a job shape and the device.
+static int check_params(const rs_ctx* ctx, const rs_params* prm, const char* who) {
+  if (prm->algo != RS_ALGO_THREE_LISTS && prm->algo != RS_ALGO_MERGED)
+    return fail(RS_E_ARG, "%s: unknown algorithm %d", who, prm->algo);
+  if (prm->algo == RS_ALGO_MERGED && (prm->Q & 1))
+    return fail(RS_E_ARG, "%s: the merged table needs an even exponent span Q (got %u)", who, prm->Q);
+  if (!(prm->threads == 128 || prm->threads == 256 || prm->threads == 512 || prm->threads == 1024))
+    return fail(RS_E_ARG, "%s: threads %u not in {128,256,512,1024}", who, prm->threads);
+  const uint32_t entries = prm->algo == RS_ALGO_MERGED ? prm->Q / 2 : prm->Q;  // baby steps in one table
+  if (prm->Q < 16 || !pow2(prm->hsize) || prm->hsize <= entries || prm->Q > 65536)
+    return fail(RS_E_ARG, "%s: need Q in [16, 65536] and a power-of-two hsize above %u table entries (Q=%u hsize=%u)",
+                who, entries, prm->Q, prm->hsize);
+  if (prm->algo == RS_ALGO_THREE_LISTS && 2ull * prm->Q > 65536)
+    return fail(RS_E_ARG, "%s: QQ = %u exceeds the 16-bit table index", who, 2 * prm->Q);
+  if (bsgs_lds_bytes(prm->hsize) + 64 > ctx->lds_per_block)
+    return fail(RS_E_ARG, "%s: hash of %u slots needs %zu LDS bytes, device has %u", who, prm->hsize,
+                bsgs_lds_bytes(prm->hsize), ctx->lds_per_block);
+  return RS_OK;
+}
+
+// Apply a checked configuration to the job's device constants.
+static int apply_params(rs_job* j, const rs_params* prm) {
+  JobDev& h = j->h;
+  j->prm.algo = prm->algo;
+  j->prm.Q = prm->Q;
+  j->prm.hsize = prm->hsize;
+  j->prm.threads = prm->threads;
+  j->prm.blocks_per_cu = prm->blocks_per_cu;
+  h.algo = (uint32_t)prm->algo;
+  h.Q = prm->Q;
+  rs_params_derive(prm, h.nmin, h.nmax, &h.M, &h.QQ, &h.MM);
+  h.hsize = prm->hsize;
+  h.hmask = prm->hsize - 1;
+  h.threads = prm->threads;
+  int bpc = 0;
+  if (bsgs_blocks_per_cu(prm->threads, prm->hsize, &bpc) != hipSuccess || bpc < 1) bpc = 1;
+  if (prm->blocks_per_cu && (int)prm->blocks_per_cu < bpc) bpc = (int)prm->blocks_per_cu;
+  j->bsgs_blocks = (uint32_t)bpc * j->ctx->cus;
+  return RS_OK;
+}
+
 extern "C" int rs_job_create(rs_ctx* ctx, uint32_t base, int32_t kcount, const uint32_t* k, const int32_t* c,
                              uint32_t nmin, uint32_t nmax, const uint32_t* bitmap, uint64_t words_per_seq,
                              const rs_params* prm, rs_job** out) {
@@ -230,19 +272,10 @@ extern "C" int rs_job_create(rs_ctx* ctx, uint32_t base, int32_t kcount, const u
   if (nmin & 1u) return fail(RS_E_ARG, "rs_job_create: nmin must be even (bitmap origin)");
   if (words_per_seq < (((uint64_t)nmax - nmin + 1) + 31) / 32)
     return fail(RS_E_ARG, "rs_job_create: words_per_seq too small for [%u, %u]", nmin, nmax);
-  if (prm->algo != RS_ALGO_THREE_LISTS && prm->algo != RS_ALGO_MERGED)
-    return fail(RS_E_ARG, "rs_job_create: unknown algorithm %d", prm->algo);
-  if (prm->algo == RS_ALGO_MERGED && (prm->Q & 1))
-    return fail(RS_E_ARG, "rs_job_create: the merged table needs an even exponent span Q (got %u)", prm->Q);
-  if (!(prm->threads == 128 || prm->threads == 256 || prm->threads == 512 || prm->threads == 1024))
-    return fail(RS_E_ARG, "rs_job_create: threads %u not in {128,256,512,1024}", prm->threads);
-  const uint32_t entries = prm->algo == RS_ALGO_MERGED ? prm->Q / 2 : prm->Q;  // baby steps in one table
-  if (prm->Q < 16 || !pow2(prm->hsize) || prm->hsize <= entries || prm->Q > 65536)
-    return fail(RS_E_ARG, "rs_job_create: need Q in [16, 65536] and a power-of-two hsize above %u table entries "
-                "(Q=%u hsize=%u)", entries, prm->Q, prm->hsize);
-  if (bsgs_lds_bytes(prm->hsize) + 64 > ctx->lds_per_block)
-    return fail(RS_E_ARG, "rs_job_create: hash of %u slots needs %zu LDS bytes, device has %u", prm->hsize,
-                bsgs_lds_bytes(prm->hsize), ctx->lds_per_block);
+  {
+    const int rc = check_params(ctx, prm, "rs_job_create");
+    if (rc) return rc;
+  }
   for (int i = 0; i < kcount; i++) {
     if (k[i] == 0) return fail(RS_E_ARG, "rs_job_create: k[%d] = 0", i);
     if (c[i] != 1 && c[i] != -1) return fail(RS_E_ARG, "rs_job_create: c[%d] = %d, need +1 or -1", i, c[i]);
@@ -273,8 +306,6 @@ extern "C" int rs_job_create(rs_ctx* ctx, uint32_t base, int32_t kcount, const u
       h.cpos[i] = c[i] > 0;
     }
   }
-  h.algo = (uint32_t)prm->algo;
-  h.Q = prm->Q;
   {
     rs_params ref;
     if (rs_params_reference(nmin, nmax, kcount, 65536, ctx->cus, &ref) != RS_OK) {
@@ -283,23 +314,11 @@ extern "C" int rs_job_create(rs_ctx* ctx, uint32_t base, int32_t kcount, const u
     }
     h.Qref = ref.Q;
   }
-  rs_params_derive(prm, nmin, nmax, &h.M, &h.QQ, &h.MM);
-  h.hsize = prm->hsize;
-  h.hmask = prm->hsize - 1;
-  h.threads = prm->threads;
   h.record_factors = prm->record_factors ? 1 : 0;
   h.factor_cap = prm->factor_cap ? prm->factor_cap : 1;
-  if (prm->algo == RS_ALGO_THREE_LISTS && h.QQ > 65536) {
-    job_free(j);
-    return fail(RS_E_ARG, "rs_job_create: QQ = %u exceeds the 16-bit table index", h.QQ);
-  }
+  apply_params(j, prm);
   j->nwords = (uint64_t)kcount * words_per_seq;
   j->seg_cap = std::max<uint32_t>(4096, 2 * (prm->seg_primes ? prm->seg_primes : 65536));
-
-  int bpc = 0;
-  if (bsgs_blocks_per_cu(prm->threads, prm->hsize, &bpc) != hipSuccess || bpc < 1) bpc = 1;
-  if (prm->blocks_per_cu && (int)prm->blocks_per_cu < bpc) bpc = (int)prm->blocks_per_cu;
-  j->bsgs_blocks = (uint32_t)bpc * ctx->cus;
 
 #define ALLOC(ptr, bytes)                                                                        \
   do {                                                                                           \
@@ -316,7 +335,7 @@ extern "C" int rs_job_create(rs_ctx* ctx, uint32_t base, int32_t kcount, const u
   ALLOC(j->d_bs, sizeof(BatchState));
   ALLOC(j->dprimes, (size_t)j->seg_cap * 8);
   ALLOC(j->drecs, (size_t)j->seg_cap * sizeof(PrimeRec));
-  ALLOC(j->dkrecs, (size_t)j->seg_cap * kcount * (prm->algo == RS_ALGO_MERGED ? 2 : 1) * sizeof(KRec));
+  ALLOC(j->dkrecs, (size_t)j->seg_cap * kcount * 2 * sizeof(KRec));  // room for the merged table's targets
 #undef ALLOC
   SegJob& sj = j->h_sj;
   memset(&sj, 0, sizeof(sj));
@@ -645,4 +664,147 @@ extern "C" int rs_primegen(rs_ctx* ctx, uint64_t pmin, uint64_t pmax, uint64_t* 
   (void)hipFree(dbs);
   *count = total;
   return rc;
+}
+
+extern "C" int rs_context_stream(rs_ctx* ctx, void** stream) {
+  if (!ctx || !stream) return fail(RS_E_ARG, "rs_context_stream: null");
+  *stream = (void*)ctx->stream;
+  return RS_OK;
+}
+
+extern "C" int rs_job_reconfigure(rs_job* j, const rs_params* prm) {
+  if (!j || !prm) return fail(RS_E_ARG, "rs_job_reconfigure: null");
+  int rc = check_params(j->ctx, prm, "rs_job_reconfigure");
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(j->ctx->device));
+  HIPCHK(hipStreamSynchronize(j->ctx->stream));
+  const uint32_t want = std::max<uint32_t>(4096, 2 * (prm->seg_primes ? prm->seg_primes : 65536));
+  if (want > j->seg_cap) {
+    uint64_t* np = nullptr;
+    PrimeRec* nr = nullptr;
+    KRec* nk = nullptr;
+    if (hipMalloc((void**)&np, (size_t)want * 8) != hipSuccess ||
+        hipMalloc((void**)&nr, (size_t)want * sizeof(PrimeRec)) != hipSuccess ||
+        hipMalloc((void**)&nk, (size_t)want * j->h.kcount * 2 * sizeof(KRec)) != hipSuccess) {
+      (void)hipFree(np);
+      (void)hipFree(nr);
+      return fail(RS_E_NOMEM, "rs_job_reconfigure: segment buffers");
+    }
+    (void)hipFree(j->dprimes);
+    (void)hipFree(j->drecs);
+    (void)hipFree(j->dkrecs);
+    j->dprimes = np;
+    j->drecs = nr;
+    j->dkrecs = nk;
+    j->seg_cap = want;
+    j->h_sj.primes = np;
+    j->h_sj.recs = nr;
+    j->h_sj.krecs = nk;
+    j->h_sj.cap = want;
+    HIPCHK(hipMemcpy(j->d_sj, &j->h_sj, sizeof(SegJob), hipMemcpyHostToDevice));
+  }
+  j->prm.seg_primes = prm->seg_primes;
+  apply_params(j, prm);
+  HIPCHK(hipMemcpy(j->djob, &j->h, sizeof(JobDev), hipMemcpyHostToDevice));
+  return RS_OK;
+}
+
+// The fixed candidate set of the tuner and the cost model: slot 0 is the
+// reference split; then the merged table at 1024..8192 entries, hash load
+// 1/2 or 1/4, and 256, 512 or 1024 threads per workgroup.
+extern "C" int rs_tune_candidates(rs_ctx* ctx, uint32_t nmin, uint32_t nmax, int32_t kcount, rs_params* out, int cap,
+                                  int* n) {
+  if (!ctx || !out || !n || cap < 1) return fail(RS_E_ARG, "rs_tune_candidates: bad args");
+  rs_params ref;
+  int rc = rs_params_reference(nmin, nmax, kcount, 65536, ctx->cus, &ref);
+  if (rc) return rc;
+  int m = 0;
+  out[m++] = ref;
+  const uint32_t entries[4] = {1024, 2048, 4096, 8192};
+  const uint32_t load[2] = {2, 4};
+  const uint32_t threads[3] = {256, 512, 1024};
+  for (uint32_t e : entries)
+    for (uint32_t l : load)
+      for (uint32_t t : threads) {
+        rs_params p = ref;
+        p.algo = RS_ALGO_MERGED;
+        p.Q = 2 * e;
+        p.hsize = e * l;
+        p.threads = t;
+        if (check_params(ctx, &p, "rs_tune_candidates") != RS_OK) continue;
+        if (m < cap) out[m] = p;
+        m++;
+      }
+  *n = m;
+  rs::set_error("");
+  return RS_OK;
+}
+
+extern "C" int rs_autotune(rs_job* j, uint64_t pmin, uint64_t width, const rs_params* cand, int ncand, double* seconds,
+                           uint64_t* primes, int* best) {
+  if (!j || !cand || ncand < 1 || !seconds || !primes || !best || width == 0 || pmin < 3)
+    return fail(RS_E_ARG, "rs_autotune: bad args");
+  HIPCHK(hipSetDevice(j->ctx->device));
+  hipStream_t s = j->ctx->stream;
+  HIPCHK(hipStreamSynchronize(s));
+  // snapshot the state the samples change
+  uint32_t* saved_bm = nullptr;
+  HIPCHK(hipMalloc((void**)&saved_bm, j->nwords * 4));
+  Counters saved_cnt;
+  int rc = RS_OK;
+  if (hipMemcpy(saved_bm, j->dbitmap, j->nwords * 4, hipMemcpyDeviceToDevice) != hipSuccess ||
+      hipMemcpy(&saved_cnt, j->dcnt, sizeof(Counters), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(RS_E_HIP, "rs_autotune: snapshot");
+  const rs_params orig = j->prm;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(RS_E_HIP, "rs_autotune: events");
+  int ib = -1;
+  double tb = 0;
+  for (int i = -1; i < ncand && !rc; i++) {
+    // i == -1: one untimed pass with the first candidate (code-object load, caches)
+    rc = rs_job_reconfigure(j, &cand[i < 0 ? 0 : i]);
+    if (rc) {
+      if (i >= 0) {
+        seconds[i] = -1.0;
+        primes[i] = 0;
+        rc = RS_OK;
+        continue;
+      }
+      break;
+    }
+    const uint64_t pmax = (pmin + width < pmin) ? ~0ull : pmin + width;
+    if (hipEventRecord(e0, s) != hipSuccess) rc = fail(RS_E_HIP, "rs_autotune: event");
+    if (!rc) rc = rs_sieve(j, pmin, pmax);
+    if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+      rc = fail(RS_E_HIP, "rs_autotune: sync");
+    Counters c;
+    if (!rc && hipMemcpy(&c, j->dcnt, sizeof(Counters), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(RS_E_HIP, "rs_autotune: counters");
+    float ms = 0;
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(RS_E_HIP, "rs_autotune: timing");
+    // restore
+    if (!rc && (hipMemcpy(j->dbitmap, saved_bm, j->nwords * 4, hipMemcpyDeviceToDevice) != hipSuccess ||
+                hipMemcpy(j->dcnt, &saved_cnt, sizeof(Counters), hipMemcpyHostToDevice) != hipSuccess))
+      rc = fail(RS_E_HIP, "rs_autotune: restore");
+    if (rc || i < 0) continue;
+    seconds[i] = ms * 1e-3;
+    primes[i] = c.primecount - saved_cnt.primecount;
+    if (ib < 0 || seconds[i] < tb) {
+      ib = i;
+      tb = seconds[i];
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(saved_bm);
+  j->ev_used = 0;
+  j->timer_started = false;
+  if (rc) {
+    rs_job_reconfigure(j, &orig);
+    return rc;
+  }
+  *best = ib;
+  if (ib >= 0) return rs_job_reconfigure(j, &cand[ib]);
+  return rs_job_reconfigure(j, &orig);
 }

@@ -1,0 +1,206 @@
+"""Configuration search for sieve jobs: the autotuner's timing records and the
+learned cost model built on them.
+
+  descriptor      (base, #sequences, #terms, n-span, log2 p) of a job
+  collect()       for each descriptor, a synthetic job of that shape is timed
+                  by the library's autotuner (rs_autotune) on every candidate
+                  configuration (rs_tune_candidates): one record per job with
+                  the seconds per prime of each candidate
+  CostModel       the MLP of rssieve_mlp.h (5 -> 64 -> 64 -> #candidates,
+                  fp32, trained and run by the library's HIP kernels) fitted to
+                  log(seconds per prime); it picks the candidate with the
+                  lowest prediction, and falls back to the reference split
+                  (candidate 0) when it has no model
+  regret()        time of the chosen candidate over the best measured one
+
+Command line (on a GPU):
+  python -m rssieve_amd.tuning collect --n 1024 --out records.npz
+  python -m rssieve_amd.tuning evaluate --records records.npz
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+import rssieve_amd as R
+
+BASES = (2, 3, 5, 6, 10)
+N_FEATURES = 5
+
+
+def features(base, kcount, nterms, span, log2p):
+    """The five model inputs (all logarithmic)."""
+    return np.array([math.log2(base), math.log2(kcount), math.log2(nterms + 1.0), math.log2(span), log2p],
+                    np.float32)
+
+
+def draw_descriptors(n, seed, log2p_range=(32.0, 62.0)):
+    """Job shapes: base from {2,3,5,6,10}, 1-100 sequences and n-spans
+    10^4..10^7 log-uniform, term density 1-10 % (what a pre-sieve leaves),
+    p from 2^32..2^62 log-uniform."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        base = int(rng.choice(BASES))
+        K = int(round(math.exp(rng.uniform(0.0, math.log(100.0)))))
+        K = max(1, min(100, K))
+        span = int(round(math.exp(rng.uniform(math.log(1e4), math.log(1e7)))))
+        density = float(math.exp(rng.uniform(math.log(0.01), math.log(0.10))))
+        log2p = float(rng.uniform(*log2p_range))
+        nmin = int(rng.integers(1000, 10**7)) & ~1
+        out.append({"base": base, "K": K, "span": span, "density": density, "log2p": log2p, "nmin": nmin,
+                    "seed": int(rng.integers(0, 2**31))})
+    return out
+
+
+def synth_job(ctx, d, params):
+    """A job of descriptor d: seeded k coprime to the base (odd for even bases,
+    even for odd ones), c alternating, a random bitmap whose bits are set
+    with probability 2^-round(-log2 density)."""
+    rng = np.random.default_rng(d["seed"])
+    base, K = d["base"], d["K"]
+    ks = []
+    while len(ks) < K:
+        k = int(rng.integers(3, 2**31))
+        if (k % 2 == 0) == (base % 2 == 0) or math.gcd(k, base) != 1 or k in ks:
+            continue
+        ks.append(k)
+    cs = [1 if i % 2 else -1 for i in range(K)]
+    nmin = d["nmin"]
+    nmax = nmin + d["span"] - 1
+    wps = (nmax - nmin + 1 + 31) // 32
+    # every bit set with probability 2^-r ~ density: the AND of r random words
+    nwords = K * wps
+    r = max(1, int(round(-math.log2(d["density"]))))
+    bits = rng.integers(0, 2**32, size=nwords, dtype=np.uint32)
+    for _ in range(r - 1):
+        bits &= rng.integers(0, 2**32, size=nwords, dtype=np.uint32)
+    tail = (nmax - nmin + 1) % 32
+    if tail:
+        bits.reshape(K, wps)[:, -1] &= np.uint32((1 << tail) - 1)
+    nterms = int(np.bitwise_count(bits).sum())
+    return ctx.job(base, ks, cs, nmin, nmax, bits, wps, params), nterms, nmin, nmax
+
+
+def sample_width(log2p, primes=65536):
+    """p-range holding about 'primes' primes at 2^log2p."""
+    return int(primes * log2p * math.log(2.0))
+
+
+def collect(ctx, descs, log=None):
+    """Time every candidate for every descriptor; returns (X, seconds per
+    prime [n x C], valid mask, candidate list of the first job)."""
+    X, Y, V = [], [], []
+    cand0 = None
+    t0 = time.time()
+    for i, d in enumerate(descs):
+        prm = R.reference_params(d["nmin"], d["nmin"] + d["span"] - 1, d["K"], 65536, ctx.compute_units)
+        job, nterms, nmin, nmax = synth_job(ctx, d, prm)
+        cands = ctx.tune_candidates(nmin, nmax, d["K"])
+        if cand0 is None:
+            cand0 = cands
+        pmin = int(2 ** d["log2p"])
+        width = sample_width(d["log2p"])
+        best, secs, primes = job.autotune(pmin, width, cands)
+        job.close()
+        sp = np.array([s / max(p, 1) if s > 0 else np.nan for s, p in zip(secs, primes)], np.float64)
+        X.append(features(d["base"], d["K"], nterms, d["span"], d["log2p"]))
+        Y.append(sp)
+        V.append(np.isfinite(sp))
+        if log and (i % 32 == 0 or i == len(descs) - 1):
+            print("collect %d/%d  %.1fs  best=%d  sec/prime best %.3g ref %.3g" % (
+                i + 1, len(descs), time.time() - t0, best, np.nanmin(sp), sp[0]), file=log, flush=True)
+    return np.array(X), np.array(Y), np.array(V), cand0
+
+
+class CostModel:
+    """log(seconds per prime) of every candidate from the five features."""
+
+    def __init__(self, ctx, n_out, hidden=64, seed=1):
+        self.ctx = ctx
+        self.mlp = ctx.mlp(N_FEATURES, hidden, n_out, seed=seed)
+        self.mu = np.zeros(N_FEATURES, np.float32)
+        self.sd = np.ones(N_FEATURES, np.float32)
+        self.ymu = 0.0
+        self.trained = False
+
+    def _norm(self, X):
+        return ((np.asarray(X, np.float32) - self.mu) / self.sd).astype(np.float32)
+
+    def fit(self, X, Y, V, epochs=400, batch=128, lr=2e-3, seed=1):
+        self.mu = X.mean(0).astype(np.float32)
+        self.sd = (X.std(0) + 1e-6).astype(np.float32)
+        T = np.log(np.where(V, Y, 1.0))
+        self.ymu = float(T[V].mean())
+        T = np.where(V, T - self.ymu, 0.0).astype(np.float32)
+        loss = self.mlp.train(self._norm(X), T, V.astype(np.float32), epochs=epochs, batch=batch, lr=lr, seed=seed)
+        self.trained = True
+        return loss
+
+    def predict(self, X):
+        return self.mlp.forward(self._norm(X)) + self.ymu
+
+    def choose(self, X, valid=None):
+        """Index of the predicted-fastest candidate per row (0, the reference
+        split, when the model is untrained)."""
+        if not self.trained:
+            return np.zeros(len(X), np.int64)
+        P = self.predict(X)
+        if valid is not None:
+            P = np.where(valid, P, np.inf)
+        return np.argmin(P, axis=1)
+
+
+def regret(Y, V, choice):
+    """time(chosen) / time(best measured) - 1, per row."""
+    Yv = np.where(V, Y, np.inf)
+    best = Yv.min(1)
+    got = Yv[np.arange(len(Y)), choice]
+    return got / best - 1.0
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("collect")
+    c.add_argument("--n", type=int, default=1024)
+    c.add_argument("--seed", type=int, default=4096)
+    c.add_argument("--out", required=True)
+    e = sub.add_parser("evaluate")
+    e.add_argument("--records", required=True)
+    e.add_argument("--test", type=int, default=256)
+    e.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    ctx = R.Context(0)
+    if a.cmd == "collect":
+        descs = draw_descriptors(a.n, a.seed)
+        X, Y, V, cands = collect(ctx, descs, log=sys.stdout)
+        np.savez_compressed(a.out, X=X, Y=Y, V=V, descs=json.dumps(descs),
+                            cands=json.dumps([p.as_dict() for p in cands]))
+        print("wrote", a.out, X.shape, Y.shape)
+    else:
+        z = np.load(a.records, allow_pickle=False)
+        X, Y, V = z["X"], z["Y"], z["V"]
+        ntr = len(X) - a.test
+        m = CostModel(ctx, Y.shape[1])
+        loss = m.fit(X[:ntr], Y[:ntr], V[:ntr])
+        ch = m.choose(X[ntr:], V[ntr:])
+        r_model = regret(Y[ntr:], V[ntr:], ch)
+        r_ref = regret(Y[ntr:], V[ntr:], np.zeros(len(ch), np.int64))
+        res = {"train_rows": int(ntr), "test_rows": int(len(ch)), "final_loss": float(loss),
+               "regret_model_mean": float(r_model.mean()), "regret_model_median": float(np.median(r_model)),
+               "regret_reference_mean": float(r_ref.mean()), "regret_reference_median": float(np.median(r_ref)),
+               "speedup_model_vs_reference_geomean": float(np.exp(np.mean(np.log((1 + r_ref) / (1 + r_model)))))}
+        print(json.dumps(res))
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump(res, f)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

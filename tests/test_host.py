@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def test_library_exports_every_declared_symbol():
     names = set()
-    for h in ("rssieve.h", "rssieve_io.h"):
+    for h in ("rssieve.h", "rssieve_io.h", "rssieve_mlp.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         names |= set(re.findall(r"\b(rs_[a-z0-9_]+)\s*\(", text))
