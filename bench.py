@@ -8,11 +8,18 @@ n in [3*10^6, 10^7) with every term that has a prime factor < 2^20 removed
 consecutive slices of p in [2^50, 2^50 + 2^38).  One step = one slice of
 2^28 integers (~7.7 M primes); 'primes tested' is the reference's 'prime
 count' (2-PRPs generated, setup.cl:99-101).  Inputs are resident in HBM before
-the timed region; the timed region is warmup-free, bracketed by device
-synchronisation.
+the timed regions, which are bracketed by device synchronisation.
 
-Prints one JSON line (see the keys below).  --gpus N > 1 exits: this build
-runs on one GPU.
+Two paths sieve the same slices of the same terms:
+  plain  the reference's BSGS split (three tables, Q from build_giant_kernels,
+         256-thread workgroups), one job, one round per segment;
+  tuned  the configuration chosen for this job (the learned cost model when
+         a trained model is committed, else the autotuner), the job split
+         into 'batch' views sieved as one batch: one persistent BSGS launch
+         per round over all of them, rounds replayed from a HIP graph.
+Their final bitmaps must be identical (checked).  'value' is the tuned path.
+
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
 import argparse
 import json
@@ -34,8 +41,10 @@ SLICE = 2**28
 K = 100
 SEED = 2026
 PRESIEVE = 2**20
-# MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 3.93e13 VALU lane-ops/s.
+# MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 3.93e13 VALU lane-ops/s;
+# v_mad_u64_u32 (one 32x32->64 product) issues at half that rate.
 VALU_LANE_OPS = 256 * 64 * 2.4e9
+MAD_PEAK = VALU_LANE_OPS / 2
 # 32x32->64 products per 64-bit Montgomery product (4 for a*b, 3 for m=lo*q,
 # 4 for hi(m*p)).
 PRODUCTS_PER_MMUL = 11
@@ -52,7 +61,6 @@ def make_workload():
     nmin = NMIN & ~1
     wps = (NMAX - nmin + 1 + 31) // 32
     bits = np.zeros(K * wps, np.uint32)
-    # every n in [NMIN, NMAX] present before the pre-sieve
     full = np.full(wps, 0xFFFFFFFF, np.uint32)
     tail = (NMAX - nmin + 1) % 32
     if tail:
@@ -71,7 +79,7 @@ def cpu_baseline(ks, cs, nmin, nmax, bits, target_seconds=15.0):
     import oracle as O
     nthreads = max(1, min(16, os.cpu_count() or 1))
     job = O.make_job(2, ks, cs, nmin, nmax)
-    lo = P0 + PRANGE - 2**30  # a slice the GPU steps do not reach first
+    lo = P0 + PRANGE - 2**30  # a slice the GPU steps do not reach
     width = 2**18
     spent, primes, total_w = 0.0, 0, 0
     while spent < target_seconds and total_w < 2**30:
@@ -89,58 +97,148 @@ def cpu_baseline(ks, cs, nmin, nmax, bits, target_seconds=15.0):
             "sample": "oracle on p in [2^50+2^38-2^30, +%d): %d primes, %.1f s" % (total_w, primes, spent)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the sieve shards by p-range, "
-              "see DESIGN.md)" % args.gpus, file=sys.stderr)
-        sys.exit(2)
-    if args.steps + args.warmup > PRANGE // SLICE:
-        sys.exit("bench.py: at most %d steps fit the p-range" % (PRANGE // SLICE))
+def roofline(work_mmul, bsgs_s, launches, kernel):
+    avg = bsgs_s / max(launches, 1)
+    achieved = work_mmul * PRODUCTS_PER_MMUL / max(launches, 1) / max(avg, 1e-12) / 1e12
+    peak = MAD_PEAK / 1e12
+    return {"bound": "valu-int-mul", "achieved": achieved, "peak": peak, "unit": "Tproducts/s",
+            "frac": achieved / peak, "traffic": None, "kernel": kernel, "launches": launches,
+            "avg_launch_ms": avg * 1e3, "mmul_per_launch": work_mmul / max(launches, 1)}
 
-    t_setup = time.perf_counter()
-    ks, cs, nmin, nmax, bits, wps = make_workload()
-    nterms = int(np.unpackbits(bits.view(np.uint8)).sum())
-    setup_s = time.perf_counter() - t_setup
 
-    ctx = R.Context(0)
+def run_plain(ctx, data, steps, warmup):
+    ks, cs, nmin, nmax, bits, wps = data
     prm = R.reference_params(nmin, nmax, K, 65536, ctx.compute_units)
     prm.factor_cap = 1 << 20
     job = ctx.job(2, ks, cs, nmin, nmax, bits, wps, prm)
-    M, QQ, MM = R.derive(prm, nmin, nmax)
-
     p = P0
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         job.sieve(p, p + SLICE)
         p += SLICE
-    c0 = job.counters()  # synchronises
+    c0 = job.counters()
     job.timer_reset()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         job.sieve(p, p + SLICE)
         p += SLICE
     ctx.sync()
     t1 = time.perf_counter()
     c1 = job.counters()
-    # BSGS launch durations over the timed region, from HIP events recorded on
-    # the stream the kernels run on
-    bsgs_s, launches, tot_s = job.kernel_time()
+    bsgs_s, launches, _ = job.kernel_time()
+    bm = job.bitmap()
     job.drain(1 << 20)
     job.close()
-    ctx.close()
+    M, QQ, MM = R.derive(prm, nmin, nmax)
+    return {"primes": c1.primecount - c0.primecount, "seconds": t1 - t0, "bitmap": bm,
+            "roof": roofline(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % prm.threads),
+            "busy": bsgs_s / (t1 - t0), "claims": c1.claims, "dups": c1.dups,
+            "cfg": {"algo": "three lists", "Q": prm.Q, "M": M, "QQ": QQ, "MM": MM, "hsize": prm.hsize,
+                    "threads": prm.threads, "seg_primes": prm.seg_primes}}
 
-    elapsed = t1 - t0
-    primes = c1.primecount - c0.primecount
-    value = primes / elapsed
-    work = (c1.work_mmul - c0.work_mmul) * PRODUCTS_PER_MMUL
-    avg_launch = bsgs_s / max(launches, 1)
-    achieved = work / max(launches, 1) / avg_launch / 1e12
-    peak = VALU_LANE_OPS / 2 / 1e12  # v_mad_u64_u32: one 32x32->64 product per lane every 2 issue cycles
+
+def choose_config(ctx, job, data, log):
+    """The tuned configuration: the committed cost model's pick when one is
+    available, else the autotuner's measured best on one sample slice."""
+    ks, cs, nmin, nmax, bits, wps = data
+    cands = ctx.tune_candidates(nmin, nmax, K)
+    how = "autotune"
+    try:
+        from rssieve_amd import tuning
+        model = tuning.load_model(ctx)
+    except Exception as e:  # no committed model: fall back to measuring
+        model = None
+        log["model_error"] = str(e)[:200]
+    if model is not None and len(cands) == model.n_out:
+        nterms = int(np.bitwise_count(bits).sum())
+        x = tuning.features(2, K, nterms, nmax - nmin + 1, np.log2(float(P0)))[None, :]
+        idx = int(model.choose(x)[0])
+        how = "cost model"
+        log["model_pick"] = idx
+    else:
+        best, secs, primes = job.autotune(P0 + PRANGE - 2**31, 2**26, cands)
+        idx = best
+        log["autotune_seconds"] = [round(x, 5) for x in secs]
+    return cands[idx], how, idx
+
+
+def run_tuned(ctx, data, steps, warmup, batch):
+    ks, cs, nmin, nmax, bits, wps = data
+    prm = R.reference_params(nmin, nmax, K, 65536, ctx.compute_units)
+    prm.factor_cap = 1 << 20
+    job = ctx.job(2, ks, cs, nmin, nmax, bits, wps, prm)
+    log = {}
+    cfg, how, idx = choose_config(ctx, job, data, log)
+    cfg.factor_cap = 1 << 20
+    job.reconfigure(cfg)
+    views = [job.view() for _ in range(batch - 1)]
+    handles = [job] + views
+    part = SLICE // batch
+
+    def step(p):
+        R.sieve_batch(handles, [p + i * part for i in range(batch)],
+                      [p + (i + 1) * part if i < batch - 1 else p + SLICE for i in range(batch)])
+
+    p = P0
+    for _ in range(warmup):
+        step(p)
+        p += SLICE
+    c0 = job.counters()
+    job.timer_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(p)
+        p += SLICE
+    ctx.sync()
+    t1 = time.perf_counter()
+    c1 = job.counters()
+    bsgs_s, launches, _ = job.kernel_time()
+    bm = job.bitmap()
+    job.drain(1 << 20)
+    for v in views:
+        v.close()
+    job.close()
+    return {"primes": c1.primecount - c0.primecount, "seconds": t1 - t0, "bitmap": bm,
+            "roof": roofline(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % cfg.threads),
+            "busy": bsgs_s / (t1 - t0), "claims": c1.claims, "dups": c1.dups,
+            "cfg": {"algo": "merged" if cfg.algo == R.ALGO_MERGED else "three lists", "Q": cfg.Q,
+                    "hsize": cfg.hsize, "threads": cfg.threads, "seg_primes": cfg.seg_primes, "batch": batch,
+                    "chosen_by": how, "candidate": idx},
+            "log": log}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=2, help="views of the job sieved per batched round")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--plain-only", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the sieve shards by p-range, "
+              "see DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    if args.steps + args.warmup > (PRANGE - 2**31) // SLICE:
+        sys.exit("bench.py: too many steps for the p-range")
+
+    t_setup = time.perf_counter()
+    data = make_workload()
+    nterms = int(np.bitwise_count(data[4]).sum())
+    setup_s = time.perf_counter() - t_setup
+
+    ctx = R.Context(0)
+    plain = run_plain(ctx, data, args.steps, args.warmup)
+    plain_value = plain["primes"] / plain["seconds"]
+    res = plain
+    tuned = None
+    if not args.plain_only:
+        tuned = run_tuned(ctx, data, args.steps, args.warmup, args.batch)
+        if not np.array_equal(tuned["bitmap"], plain["bitmap"]):
+            sys.exit("bench.py: tuned and plain paths eliminated different terms")
+        res = tuned
+    ctx.close()
+    value = res["primes"] / res["seconds"]
     line = {
         "metric": "primes tested/sec, 100 seqs p~2^50: batched+learned-tuned vs plain GPU, same terms",
         "value": value,
@@ -148,7 +246,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": res["seconds"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -156,21 +254,21 @@ def main():
         "data": "synthetic",
         "config": {"workload": "b=2, K=100 odd k<2^32 (seed 2026), n in [3e6,1e7) pre-sieved to 2^20 "
                                "(%d terms), p slices of 2^28 from 2^50" % nterms,
-                   "path": "plain GPU sieve (reference BSGS split)",
-                   "Q": prm.Q, "M": M, "QQ": QQ, "MM": MM, "hsize": prm.hsize, "threads": prm.threads,
-                   "seg_primes": prm.seg_primes},
-        "roofline": {"bound": "valu-int-mul", "achieved": achieved, "peak": peak, "unit": "Tproducts/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": "k_bsgs3<256>", "launches": launches, "avg_launch_ms": avg_launch * 1e3,
-                     "mmul_per_launch": (c1.work_mmul - c0.work_mmul) / max(launches, 1),
-                     "bsgs_busy_frac": bsgs_s / elapsed},
-        "plain_value": value,
-        "primes_per_step": primes / args.steps,
-        "factors": int(c1.claims), "duplicates": int(c1.dups),
+                   "path": "tuned+batched" if tuned else "plain", **res["cfg"]},
+        "roofline": dict(res["roof"], bsgs_busy_frac=res["busy"]),
+        "plain_value": plain_value,
+        "plain_config": plain["cfg"],
+        "plain_roofline": dict(plain["roof"], bsgs_busy_frac=plain["busy"]),
+        "tuned_over_plain": value / plain_value,
+        "same_terms": tuned is not None,
+        "primes_per_step": res["primes"] / args.steps,
+        "factors": int(res["claims"]),
         "setup_s": setup_s,
     }
+    if tuned:
+        line["tuned_log"] = tuned["log"]
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(ks, cs, nmin, nmax, bits)
+        line["cpu_baseline"] = cpu_baseline(*data[:5])
     print(json.dumps(line))
 
 

@@ -118,6 +118,12 @@ int rs_params_derive(const rs_params *prm, uint32_t nmin, uint32_t nmax, uint32_
 int rs_job_create(rs_ctx *ctx, uint32_t base, int32_t kcount, const uint32_t *k, const int32_t *c, uint32_t nmin,
                   uint32_t nmax, const uint32_t *bitmap, uint64_t words_per_seq, const rs_params *prm, rs_job **out);
 void rs_job_destroy(rs_job *job);
+/* A second handle on the same job: it shares the job's sequences,
+ * configuration, bitmap, counters and factor list, and has its own segment
+ * buffers, so rs_sieve_batch can sieve several p-ranges of one job in one
+ * launch ("batch split").  Destroy views before their job; a view cannot be
+ * reconfigured.  No reference counterpart. */
+int rs_job_view(rs_job *job, rs_job **view);
 /* k, c of sequence kidx as reported in factor lines (the signed-int encoding
  * when k31_compat is set, cl_sieve.cpp:604-605). */
 int rs_job_sequence(const rs_job *job, uint32_t kidx, uint32_t *k, int32_t *c);

@@ -95,6 +95,7 @@ _SIGS = {
                                 C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(Params),
                                 C.POINTER(C.c_void_p)]),
     "rs_job_destroy": (None, [C.c_void_p]),
+    "rs_job_view": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "rs_job_sequence": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]),
     "rs_sieve": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
     "rs_sieve_batch": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
@@ -384,6 +385,17 @@ class Job:
         t, n, tot = C.c_double(), C.c_uint64(), C.c_double()
         _check(lib().rs_job_kernel_time(self._h, C.byref(t), C.byref(n), C.byref(tot)))
         return t.value, n.value, tot.value
+
+    def view(self):
+        """A handle sharing this job's state, with its own segment buffers."""
+        v = Job.__new__(Job)
+        v.ctx, v.k, v.c, v.base = self.ctx, self.k, self.c, self.base
+        v.nmin, v.nmax, v.words_per_seq, v.nwords, v.params = (self.nmin, self.nmax, self.words_per_seq,
+                                                                self.nwords, self.params)
+        h = C.c_void_p()
+        _check(lib().rs_job_view(self._h, C.byref(h)))
+        v._h = h
+        return v
 
     def reconfigure(self, params):
         _check(lib().rs_job_reconfigure(self._h, C.byref(params)))

@@ -24,6 +24,10 @@ hipError_t launch_round(SegJob* jobs, BatchState* bs, uint32_t njobs, uint32_t g
                         uint32_t threads, uint32_t hsize, uint32_t bsgs_blocks, hipEvent_t ev0, hipEvent_t ev1,
                         hipStream_t s);
 hipError_t launch_primegen_round(SegJob* jobs, BatchState* bs, uint32_t gen_blocks, hipStream_t s);
+hipError_t launch_round_prep(SegJob* jobs, BatchState* bs, uint32_t njobs, uint32_t gen_blocks, uint32_t setup_blocks,
+                             hipStream_t s);
+hipError_t launch_round_bsgs(SegJob* jobs, BatchState* bs, uint32_t threads, uint32_t hsize, uint32_t bsgs_blocks,
+                             hipStream_t s);
 }  // namespace rs
 
 using namespace rs;
@@ -59,6 +63,10 @@ struct GraphKey {
   }
 };
 
+struct GraphExecPair {
+  hipGraphExec_t exec[2];  // round preparation, BSGS
+};
+
 struct rs_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -68,8 +76,8 @@ struct rs_ctx {
   // batch staging: SegJob array and work queue shared by rs_sieve_batch calls
   SegJob* d_bjobs = nullptr;
   BatchState* d_bstate = nullptr;
-  // one captured round per batch shape
-  std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
+  // captured rounds per batch shape
+  std::vector<std::pair<GraphKey, GraphExecPair>> graphs;
 };
 
 struct rs_job {
@@ -99,6 +107,9 @@ struct rs_job {
   hipEvent_t ev_all[2] = {nullptr, nullptr};
   bool timed = false;
   bool timer_started = false;
+  rs_job* parent = nullptr;     // set for a view: shares the parent's job constants, bitmap, counters, factors
+  double acc_bsgs_s = 0;        // BSGS time of batch rounds (graph replays)
+  uint64_t acc_launches = 0;
 };
 
 extern "C" const char* rs_last_error(void) { return g_err.c_str(); }
@@ -139,7 +150,11 @@ extern "C" void rs_close(rs_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
   }
-  for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+  for (auto& g : ctx->graphs) {
+    (void)hipGraphExecDestroy(g.second.exec[0]);
+    (void)hipGraphExecDestroy(g.second.exec[1]);
+  }
+
   (void)hipFree(ctx->d_bjobs);
   (void)hipFree(ctx->d_bstate);
   delete ctx;
@@ -197,10 +212,12 @@ extern "C" int rs_params_derive(const rs_params* prm, uint32_t nmin, uint32_t nm
 
 static void job_free(rs_job* j) {
   if (!j) return;
-  (void)hipFree(j->djob);
-  (void)hipFree(j->dbitmap);
-  (void)hipFree(j->dcnt);
-  (void)hipFree(j->dfac);
+  if (!j->parent) {
+    (void)hipFree(j->djob);
+    (void)hipFree(j->dbitmap);
+    (void)hipFree(j->dcnt);
+    (void)hipFree(j->dfac);
+  }
   (void)hipFree(j->d_sj);
   (void)hipFree(j->d_bs);
   (void)hipFree(j->dprimes);
@@ -230,12 +247,10 @@ static int check_params(const rs_ctx* ctx, const rs_params* prm, const char* who
   if (!(prm->threads == 128 || prm->threads == 256 || prm->threads == 512 || prm->threads == 1024))
     return fail(RS_E_ARG, "%s: threads %u not in {128,256,512,1024}", who, prm->threads);
   const uint32_t entries = prm->algo == RS_ALGO_MERGED ? prm->Q / 2 : prm->Q;  // baby steps in one table
-  if (prm->Q < 16 || !pow2(prm->hsize) || prm->hsize <= entries || prm->Q > 65536)
-    return fail(RS_E_ARG, "%s: need Q in [16, 65536] and a power-of-two hsize above %u table entries (Q=%u hsize=%u)",
-                who, entries, prm->Q, prm->hsize);
-  if (prm->algo == RS_ALGO_THREE_LISTS && 2ull * prm->Q > 65536)
-    return fail(RS_E_ARG, "%s: QQ = %u exceeds the 16-bit table index", who, 2 * prm->Q);
-  if (bsgs_lds_bytes(prm->hsize) + 64 > ctx->lds_per_block)
+  if (prm->Q < 16 || entries > 16384 || !pow2(prm->hsize) || prm->hsize < 64 || prm->hsize <= entries)
+    return fail(RS_E_ARG, "%s: need at most 16384 table entries and a power-of-two hsize >= 64 above them "
+                "(Q=%u entries=%u hsize=%u)", who, prm->Q, entries, prm->hsize);
+  if (bsgs_lds_bytes(prm->hsize) + 4096 > ctx->lds_per_block)
     return fail(RS_E_ARG, "%s: hash of %u slots needs %zu LDS bytes, device has %u", who, prm->hsize,
                 bsgs_lds_bytes(prm->hsize), ctx->lds_per_block);
   return RS_OK;
@@ -477,7 +492,7 @@ extern "C" int rs_sieve_batch(rs_job** jobs, int njobs, const uint64_t* pmin, co
     cap_max = std::max(cap_max, j->seg_cap);
     R = std::max(R, rounds_for(pmin[i], pmax[i], range));
   }
-  if (bsgs_lds_bytes(hsize) + 64 > ctx->lds_per_block) return fail(RS_E_ARG, "rs_sieve_batch: table too large");
+  if (bsgs_lds_bytes(hsize) + 4096 > ctx->lds_per_block) return fail(RS_E_ARG, "rs_sieve_batch: table too large");
   int bpc = 0;
   if (bsgs_blocks_per_cu(T, hsize, &bpc) != hipSuccess || bpc < 1) bpc = 1;
   const uint32_t bsgs_blocks = (uint32_t)bpc * ctx->cus;
@@ -491,21 +506,27 @@ extern "C" int rs_sieve_batch(rs_job** jobs, int njobs, const uint64_t* pmin, co
   }
   // one captured round per batch shape, replayed R times
   const GraphKey key{(uint32_t)njobs, gen_max, setup_blocks, T, hsize, bsgs_blocks};
-  hipGraphExec_t exec = nullptr;
+  // The round as two captured graphs, preparation and BSGS, with stream
+  // events between them so every replay's BSGS launch is timed.
+  GraphExecPair* gp = nullptr;
   for (auto& g : ctx->graphs)
-    if (g.first == key) exec = g.second;
-  if (!exec) {
-    hipGraph_t graph;
-    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    hipError_t e = launch_round(ctx->d_bjobs, ctx->d_bstate, (uint32_t)njobs, gen_max, setup_blocks, T, hsize,
-                                bsgs_blocks, nullptr, nullptr, s);
-    hipError_t e2 = hipStreamEndCapture(s, &graph);
-    if (e != hipSuccess || e2 != hipSuccess)
-      return fail(RS_E_HIP, "rs_sieve_batch: capture failed: %s", hipGetErrorString(e ? e : e2));
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return fail(RS_E_HIP, "rs_sieve_batch: instantiate failed: %s", hipGetErrorString(e));
-    ctx->graphs.push_back({key, exec});
+    if (g.first == key) gp = &g.second;
+  if (!gp) {
+    GraphExecPair pair{};
+    for (int c = 0; c < 2; c++) {
+      hipGraph_t graph;
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      hipError_t e = c == 0 ? launch_round_prep(ctx->d_bjobs, ctx->d_bstate, (uint32_t)njobs, gen_max, setup_blocks, s)
+                            : launch_round_bsgs(ctx->d_bjobs, ctx->d_bstate, T, hsize, bsgs_blocks, s);
+      hipError_t e2 = hipStreamEndCapture(s, &graph);
+      if (e != hipSuccess || e2 != hipSuccess)
+        return fail(RS_E_HIP, "rs_sieve_batch: capture failed: %s", hipGetErrorString(e ? e : e2));
+      e = hipGraphInstantiate(&pair.exec[c], graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return fail(RS_E_HIP, "rs_sieve_batch: instantiate failed: %s", hipGetErrorString(e));
+    }
+    ctx->graphs.push_back({key, pair});
+    gp = &ctx->graphs.back().second;
   }
   rs_job* j0 = jobs[0];
   if (!j0->timer_started) {
@@ -517,8 +538,9 @@ extern "C" int rs_sieve_batch(rs_job** jobs, int njobs, const uint64_t* pmin, co
     int rc = record_event(j0, &e0);
     if (!rc) rc = record_event(j0, &e1);
     if (rc) return rc;
+    HIPCHK(hipGraphLaunch(gp->exec[0], s));
     HIPCHK(hipEventRecord(e0, s));
-    HIPCHK(hipGraphLaunch(exec, s));
+    HIPCHK(hipGraphLaunch(gp->exec[1], s));
     HIPCHK(hipEventRecord(e1, s));
   }
   HIPCHK(hipEventRecord(j0->ev_all[1], s));
@@ -541,8 +563,8 @@ extern "C" int rs_job_kernel_time(rs_job* j, double* bsgs_s, uint64_t* launches,
     HIPCHK(hipEventElapsedTime(&ms, j->ev[i], j->ev[i + 1]));
     t += ms * 1e-3;
   }
-  if (bsgs_s) *bsgs_s = t;
-  if (launches) *launches = j->ev_used / 2;
+  if (bsgs_s) *bsgs_s = t + j->acc_bsgs_s;
+  if (launches) *launches = j->ev_used / 2 + j->acc_launches;
   if (total_s) {
     float ms = 0;
     if (j->timer_started) HIPCHK(hipEventElapsedTime(&ms, j->ev_all[0], j->ev_all[1]));
@@ -556,6 +578,8 @@ extern "C" int rs_job_timer_reset(rs_job* j) {
   HIPCHK(hipStreamSynchronize(j->ctx->stream));
   j->ev_used = 0;
   j->timer_started = false;
+  j->acc_bsgs_s = 0;
+  j->acc_launches = 0;
   return RS_OK;
 }
 
@@ -672,8 +696,51 @@ extern "C" int rs_context_stream(rs_ctx* ctx, void** stream) {
   return RS_OK;
 }
 
+extern "C" int rs_job_view(rs_job* parent, rs_job** out) {
+  if (!parent || !out) return fail(RS_E_ARG, "rs_job_view: null");
+  if (parent->parent) return fail(RS_E_ARG, "rs_job_view: a view of a view");
+  HIPCHK(hipSetDevice(parent->ctx->device));
+  rs_job* j = new rs_job;
+  j->ctx = parent->ctx;
+  j->h = parent->h;
+  j->prm = parent->prm;
+  memcpy(j->klist, parent->klist, sizeof(j->klist));
+  memcpy(j->k, parent->k, sizeof(j->k));
+  memcpy(j->c, parent->c, sizeof(j->c));
+  j->nwords = parent->nwords;
+  j->seg_cap = parent->seg_cap;
+  j->bsgs_blocks = parent->bsgs_blocks;
+  j->parent = parent;
+  j->djob = parent->djob;
+  j->dbitmap = parent->dbitmap;
+  j->dcnt = parent->dcnt;
+  j->dfac = parent->dfac;
+  const int K = parent->h.kcount;
+  if (hipMalloc((void**)&j->dprimes, (size_t)j->seg_cap * 8) != hipSuccess ||
+      hipMalloc((void**)&j->drecs, (size_t)j->seg_cap * sizeof(PrimeRec)) != hipSuccess ||
+      hipMalloc((void**)&j->dkrecs, (size_t)j->seg_cap * K * 2 * sizeof(KRec)) != hipSuccess ||
+      hipMalloc((void**)&j->d_sj, sizeof(SegJob)) != hipSuccess ||
+      hipMalloc((void**)&j->d_bs, sizeof(BatchState)) != hipSuccess ||
+      hipEventCreate(&j->ev_all[0]) != hipSuccess || hipEventCreate(&j->ev_all[1]) != hipSuccess) {
+    job_free(j);
+    return fail(RS_E_NOMEM, "rs_job_view: allocation");
+  }
+  j->h_sj = parent->h_sj;
+  j->h_sj.primes = j->dprimes;
+  j->h_sj.recs = j->drecs;
+  j->h_sj.krecs = j->dkrecs;
+  if (hipMemcpy(j->d_sj, &j->h_sj, sizeof(SegJob), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(j->d_bs, 0, sizeof(BatchState)) != hipSuccess) {
+    job_free(j);
+    return fail(RS_E_HIP, "rs_job_view: upload");
+  }
+  *out = j;
+  return RS_OK;
+}
+
 extern "C" int rs_job_reconfigure(rs_job* j, const rs_params* prm) {
   if (!j || !prm) return fail(RS_E_ARG, "rs_job_reconfigure: null");
+  if (j->parent) return fail(RS_E_ARG, "rs_job_reconfigure: reconfigure the parent job, not a view");
   int rc = check_params(j->ctx, prm, "rs_job_reconfigure");
   if (rc) return rc;
   HIPCHK(hipSetDevice(j->ctx->device));
@@ -710,8 +777,9 @@ extern "C" int rs_job_reconfigure(rs_job* j, const rs_params* prm) {
 }
 
 // The fixed candidate set of the tuner and the cost model: slot 0 is the
-// reference split; then the merged table at 1024..8192 entries, hash load
-// 1/2 or 1/4, and 256, 512 or 1024 threads per workgroup.
+// reference split; then the merged table at 1024..16384 entries, hash load
+// 1/2 or 1/4 (where the LDS holds it), and 256, 512 or 1024 threads per
+// workgroup.
 extern "C" int rs_tune_candidates(rs_ctx* ctx, uint32_t nmin, uint32_t nmax, int32_t kcount, rs_params* out, int cap,
                                   int* n) {
   if (!ctx || !out || !n || cap < 1) return fail(RS_E_ARG, "rs_tune_candidates: bad args");
@@ -720,7 +788,7 @@ extern "C" int rs_tune_candidates(rs_ctx* ctx, uint32_t nmin, uint32_t nmax, int
   if (rc) return rc;
   int m = 0;
   out[m++] = ref;
-  const uint32_t entries[4] = {1024, 2048, 4096, 8192};
+  const uint32_t entries[5] = {1024, 2048, 4096, 8192, 16384};
   const uint32_t load[2] = {2, 4};
   const uint32_t threads[3] = {256, 512, 1024};
   for (uint32_t e : entries)

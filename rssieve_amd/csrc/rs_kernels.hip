@@ -528,40 +528,86 @@ __device__ __forceinline__ void claim(const JobDev* __restrict__ job, uint32_t* 
   }
 }
 
-// Insert baby value v with exponent offset j.  A value whose low 32 bits are 0
-// is left out: the reference marks empty slots with 0 and never finds it
-// (giant.cl:44-46, 61-63).
-__device__ __forceinline__ void tab_insert(uint32_t* fp, uint16_t* ix, uint32_t hmask, uint64_t v, uint32_t j) {
+// ---------------------------------------------------------------------------
+// LDS baby-step table: buckets of 8 slots (32 bytes, two ds_read_b128 per
+// lookup, no probe loop in the common case).  A slot holds an 18-bit
+// fingerprint of the Montgomery residue (never 0) above a 14-bit table entry
+// index; 0 marks an empty slot.  The bucket comes from the residue's low bits,
+// the fingerprint from the others; a fingerprint match is confirmed by
+// recomputing the baby value, so the table answers exactly "which entry holds
+// this residue".  A residue whose low 32 bits are 0 is never inserted: the
+// reference marks empty slots with 0 and cannot find it (giant.cl:44-46, 61-63).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kBucket = 8;
+
+__device__ __forceinline__ uint32_t tab_fp(uint64_t v) {
+  const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  const uint32_t f = ((lo >> 14) ^ (hi * 0x9E3779B1u) ^ (hi >> 13)) & 0x3FFFFu;
+  return f ? f : 1u;
+}
+
+__device__ __forceinline__ void tab_insert(uint32_t* tab, uint32_t bmask, uint64_t v, uint32_t e) {
   const uint32_t lo = (uint32_t)v;
   if (lo == 0) return;
-  uint32_t s = lo & hmask;
+  const uint32_t word = (tab_fp(v) << 14) | e;
+  uint32_t b = lo & bmask;
   while (true) {
-    const uint32_t old = atomicCAS(&fp[s], 0u, lo);
-    if (old == 0u) {
-      ix[s] = (uint16_t)j;
-      return;
-    }
-    s = (s + 1) & hmask;
+    uint32_t* bk = tab + b * kBucket;
+#pragma unroll
+    for (uint32_t s = 0; s < kBucket; s++)
+      if (atomicCAS(&bk[s], 0u, word) == 0u) return;
+    b = (b + 1) & bmask;
   }
 }
 
-// Find exponent offset j with baby value == g, or -1.  Slots hold the low 32
-// bits; when p >= 2^32 a match is confirmed by recomputing b^(nstart + j).
-__device__ __forceinline__ int tab_lookup(const uint32_t* fp, const uint16_t* ix, uint32_t hmask, uint64_t g,
-                                          uint64_t mbase, uint32_t base, uint32_t nstart, uint64_t p, uint64_t q,
-                                          uint64_t one) {
-  const uint32_t lo = (uint32_t)g;
-  uint32_t s = lo & hmask;
+struct TabCtx {
+  const uint32_t* tab;
+  uint32_t bmask;
+  uint64_t mbase, p, q, one;
+  uint32_t base, nstart, stride;
+};
+
+// Entry e whose baby value b^(nstart + stride*e) equals g, or -1.
+__device__ __noinline__ int tab_find(const TabCtx& t, uint64_t g) {
+  const uint32_t fp = tab_fp(g);
+  uint32_t b = (uint32_t)g & t.bmask;
   while (true) {
-    const uint32_t f = fp[s];
-    if (f == 0u) return -1;
-    if (f == lo) {
-      const uint32_t j = ix[s];
-      if ((p >> 32) == 0) return (int)j;
-      if (basepow_sm(mbase, nstart + j, base, p, q, one) == g) return (int)j;
+    const uint4* bk = (const uint4*)(t.tab + b * kBucket);
+    const uint4 x = bk[0], y = bk[1];
+    const uint32_t w[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+    bool full = true;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      if (w[s] == 0u) full = false;
+      if ((w[s] >> 14) == fp) {
+        const uint32_t e = w[s] & 0x3FFFu;
+        if (basepow_sm(t.mbase, t.nstart + t.stride * e, t.base, t.p, t.q, t.one) == g) return (int)e;
+      }
     }
-    s = (s + 1) & hmask;
+    if (!full) return -1;
+    b = (b + 1) & t.bmask;
   }
+}
+
+// Blocked Bloom filter in front of the table: one 32-bit word per table slot,
+// two bits per baby value, both in the word chosen by the residue's low bits.
+// With at most hsize/2 values the false-positive rate is below 2^-10, so a
+// giant step costs one ds_read_b32 and a few integer ops; only the rare
+// positives consult the table.
+__device__ __forceinline__ uint32_t flt_key(uint64_t v) {
+  return (uint32_t)v ^ ((uint32_t)(v >> 32) * 0x9E3779B1u);
+}
+__device__ __forceinline__ uint32_t flt_bits(uint32_t x) {
+  return (1u << ((x >> 16) & 31u)) | (1u << ((x >> 21) & 31u));
+}
+__device__ __forceinline__ void flt_insert(uint32_t* flt, uint32_t fmask, uint64_t v) {
+  const uint32_t x = flt_key(v);
+  atomicOr(&flt[x & fmask], flt_bits(x));
+}
+__device__ __forceinline__ bool flt_maybe(const uint32_t* flt, uint32_t fmask, uint64_t v) {
+  const uint32_t x = flt_key(v);
+  const uint32_t m = flt_bits(x);
+  return (flt[x & fmask] & m) == m;
 }
 
 // One persistent kernel serves every job of a batch.  An item is (job, prime,
@@ -571,7 +617,10 @@ __device__ __forceinline__ int tab_lookup(const uint32_t* fp, const uint16_t* ix
 // table of the batch; each item uses its own job's table size.
 template <int T>
 __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchState* __restrict__ bs) {
-  extern __shared__ uint32_t lds[];
+  extern __shared__ uint4 lds4[];
+  uint32_t* tab = (uint32_t*)lds4;
+  __shared__ uint64_t s_h[2 * RS_MAXK];   // targets of the item
+  __shared__ uint32_t s_k[2 * RS_MAXK];   // sequence index | shift << 31
   __shared__ uint32_t s_item, s_job, s_order, s_dups, s_claims;
   const uint32_t tid = threadIdx.x;
   const uint32_t njobs = bs->njobs;
@@ -600,9 +649,8 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
     const PrimeRec* __restrict__ R = J.recs + pi;
     const uint32_t nk = R->nk[c];
     if (nk == 0) continue;
-    const uint32_t hmask = job->hmask, hsize = job->hsize;
-    uint32_t* fp = lds;
-    uint16_t* ix = (uint16_t*)(lds + hsize);
+    const uint32_t hsize = job->hsize, bmask = (hsize / kBucket) - 1, fmask = hsize - 1;
+    uint32_t* flt = tab + hsize;
     const uint32_t base = job->base;
     const int K = job->kcount;
     const uint32_t nmin = job->nmin, nmax = job->nmax;
@@ -620,8 +668,16 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
     uint32_t* __restrict__ bitmap = J.bitmap;
     FactorRec* __restrict__ fac = J.fac;
     Counters* __restrict__ cnt = J.cnt;
+    const TabCtx tc{tab, bmask, mbase, p, q, one, base, nstart, stride};
 
-    for (uint32_t s = tid; s < hsize; s += T) fp[s] = 0u;
+    {
+      const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+      for (uint32_t s = tid; s < hsize / 2; s += T) lds4[s] = z;  // table and filter
+    }
+    for (uint32_t i = tid; i < nk; i += T) {
+      s_h[i] = kr[i].hadj;
+      s_k[i] = kr[i].kidx | ((merged ? kr[i].cls : 0u) << 31);
+    }
     if (tid == 0) s_order = 0xFFFFFFFFu;
     __syncthreads();
     // baby steps: entry e = tid + T*i holds b^(nstart + stride*e)
@@ -631,7 +687,8 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
       for (uint32_t e = tid; e < nent; e += T) {
         const uint32_t j = e * stride;
         if (j && v == v0) atomicMin(&s_order, j);
-        tab_insert(fp, ix, hmask, v, j);
+        tab_insert(tab, bmask, v, e);
+        if ((uint32_t)v) flt_insert(flt, fmask, v);
         v = mmul(v, bstep, p, q);
       }
     }
@@ -642,38 +699,68 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
       // ord_p(b) (or its even multiple) is below the table span: keep only
       // j < order (giant.cl:121-134), then solve without giant steps.
       __syncthreads();
-      for (uint32_t s = tid; s < hsize; s += T) fp[s] = 0u;
+      {
+        const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+        for (uint32_t s = tid; s < hsize / 4; s += T) lds4[s] = z;
+      }
       __syncthreads();
       uint64_t v = vstart;
       for (uint32_t e = tid; e < nent; e += T) {
-        const uint32_t j = e * stride;
-        if (j < order) tab_insert(fp, ix, hmask, v, j);
+        if (e * stride < order) tab_insert(tab, bmask, v, e);
         v = mmul(v, bstep, p, q);
       }
       __syncthreads();
       for (uint32_t i = tid; i < nk; i += T) {
-        const int j = tab_lookup(fp, ix, hmask, kr[i].hadj, mbase, base, nstart, p, q, one);
-        if (j >= 0) {
-          const uint32_t kidx = kr[i].kidx;
-          const uint32_t d = merged ? kr[i].cls : 0;
-          for (uint64_t n = (uint64_t)nstart + (uint32_t)j + d; n <= nmax; n += order)
+        const int e = tab_find(tc, s_h[i]);
+        if (e >= 0) {
+          const uint32_t kidx = s_k[i] & 0x7FFFFFFFu, d = s_k[i] >> 31;
+          for (uint64_t n = (uint64_t)nstart + stride * (uint32_t)e + d; n <= nmax; n += order)
             claim(job, bitmap, fac, cnt, p, kidx, (uint32_t)n, &s_dups, &s_claims);
         }
       }
       work = order / stride + nk;
       probes = nk;
     } else {
-      // giant steps: thread tid handles q = tid, tid + T, ...
+      // giant steps: thread tid handles q = tid, tid + T, ...; the targets
+      // are taken four at a time so four products and four filter reads are
+      // in flight together
       const uint64_t g = R->g[cp], gT = R->gT[cp];
       uint64_t Gq = powmod_sm(g, tid, p, q, one);
       for (uint32_t qq = tid; qq < Ml; qq += T) {
         const uint32_t nq = nstart + qq * Ql;
-        for (uint32_t i = 0; i < nk; i++) {
-          const uint64_t gamma = mmul(kr[i].hadj, Gq, p, q);
-          const int j = tab_lookup(fp, ix, hmask, gamma, mbase, base, nstart, p, q, one);
-          if (j >= 0)
-            claim(job, bitmap, fac, cnt, p, kr[i].kidx, nq + (uint32_t)j + (merged ? kr[i].cls : 0u), &s_dups,
-                  &s_claims);
+        uint32_t i = 0;
+        for (; i + 4 <= nk; i += 4) {
+          const uint64_t g0 = mmul(s_h[i], Gq, p, q);
+          const uint64_t g1 = mmul(s_h[i + 1], Gq, p, q);
+          const uint64_t g2 = mmul(s_h[i + 2], Gq, p, q);
+          const uint64_t g3 = mmul(s_h[i + 3], Gq, p, q);
+          const bool m0 = flt_maybe(flt, fmask, g0), m1 = flt_maybe(flt, fmask, g1);
+          const bool m2 = flt_maybe(flt, fmask, g2), m3 = flt_maybe(flt, fmask, g3);
+          if (m0 | m1 | m2 | m3) {
+            const uint64_t gg[4] = {g0, g1, g2, g3};
+            const bool mm[4] = {m0, m1, m2, m3};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              if (!mm[u]) continue;
+              const int e = tab_find(tc, gg[u]);
+              if (e >= 0) {
+                const uint32_t kk = s_k[i + u];
+                claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu, nq + stride * (uint32_t)e + (kk >> 31), &s_dups,
+                      &s_claims);
+              }
+            }
+          }
+        }
+        for (; i < nk; i++) {
+          const uint64_t g0 = mmul(s_h[i], Gq, p, q);
+          if (flt_maybe(flt, fmask, g0)) {
+            const int e = tab_find(tc, g0);
+            if (e >= 0) {
+              const uint32_t kk = s_k[i];
+              claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu, nq + stride * (uint32_t)e + (kk >> 31), &s_dups,
+                    &s_claims);
+            }
+          }
         }
         Gq = mmul(Gq, gT, p, q);
       }
@@ -708,7 +795,7 @@ __global__ void k_round_items(SegJob* __restrict__ jobs, BatchState* __restrict_
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-size_t bsgs_lds_bytes(uint32_t hsize) { return (size_t)hsize * 6; }
+size_t bsgs_lds_bytes(uint32_t hsize) { return (size_t)hsize * 8; }  // table + filter
 
 hipError_t bsgs_blocks_per_cu(uint32_t threads, uint32_t hsize, int* out) {
   const size_t lds = bsgs_lds_bytes(hsize);
@@ -727,20 +814,20 @@ hipError_t launch_set_cursor(SegJob* sj, uint64_t pmin, uint64_t pmax, uint32_t 
   return hipGetLastError();
 }
 
-// One round of a batch: the five launches of the header comment.
-hipError_t launch_round(SegJob* jobs, BatchState* bs, uint32_t njobs, uint32_t gen_blocks, uint32_t setup_blocks,
-                        uint32_t threads, uint32_t hsize, uint32_t bsgs_blocks, hipEvent_t ev0, hipEvent_t ev1,
-                        hipStream_t s) {
+// One round of a batch in two parts: the preparation (round_begin, primegen,
+// setup, round_items) and the BSGS launch, so a caller can time the BSGS
+// launch with events between them.
+hipError_t launch_round_prep(SegJob* jobs, BatchState* bs, uint32_t njobs, uint32_t gen_blocks, uint32_t setup_blocks,
+                             hipStream_t s) {
   hipLaunchKernelGGL(k_round_begin, dim3((njobs + 63) / 64), dim3(64), 0, s, jobs, bs, njobs);
   if (gen_blocks) hipLaunchKernelGGL(k_primegen, dim3(gen_blocks, njobs), dim3(256), 0, s, jobs);
   hipLaunchKernelGGL(k_setup, dim3(setup_blocks, njobs), dim3(256), 0, s, jobs);
   hipLaunchKernelGGL(k_round_items, dim3(1), dim3(64), 0, s, jobs, bs, njobs);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  if (ev0) {
-    e = hipEventRecord(ev0, s);
-    if (e != hipSuccess) return e;
-  }
+  return hipGetLastError();
+}
+
+hipError_t launch_round_bsgs(SegJob* jobs, BatchState* bs, uint32_t threads, uint32_t hsize, uint32_t bsgs_blocks,
+                             hipStream_t s) {
   const size_t lds = bsgs_lds_bytes(hsize);
   switch (threads) {
     case 128: hipLaunchKernelGGL(k_bsgs<128>, dim3(bsgs_blocks), dim3(128), lds, s, jobs, bs); break;
@@ -749,9 +836,17 @@ hipError_t launch_round(SegJob* jobs, BatchState* bs, uint32_t njobs, uint32_t g
     case 1024: hipLaunchKernelGGL(k_bsgs<1024>, dim3(bsgs_blocks), dim3(1024), lds, s, jobs, bs); break;
     default: return hipErrorInvalidValue;
   }
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  if (ev1) e = hipEventRecord(ev1, s);
+  return hipGetLastError();
+}
+
+// One round with events around its BSGS launch (either may be null).
+hipError_t launch_round(SegJob* jobs, BatchState* bs, uint32_t njobs, uint32_t gen_blocks, uint32_t setup_blocks,
+                        uint32_t threads, uint32_t hsize, uint32_t bsgs_blocks, hipEvent_t ev0, hipEvent_t ev1,
+                        hipStream_t s) {
+  hipError_t e = launch_round_prep(jobs, bs, njobs, gen_blocks, setup_blocks, s);
+  if (e == hipSuccess && ev0) e = hipEventRecord(ev0, s);
+  if (e == hipSuccess) e = launch_round_bsgs(jobs, bs, threads, hsize, bsgs_blocks, s);
+  if (e == hipSuccess && ev1) e = hipEventRecord(ev1, s);
   return e;
 }
 

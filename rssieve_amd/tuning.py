@@ -127,6 +127,7 @@ class CostModel:
         self.sd = np.ones(N_FEATURES, np.float32)
         self.ymu = 0.0
         self.trained = False
+        self.n_out = n_out
 
     def _norm(self, X):
         return ((np.asarray(X, np.float32) - self.mu) / self.sd).astype(np.float32)
@@ -153,6 +154,29 @@ class CostModel:
         if valid is not None:
             P = np.where(valid, P, np.inf)
         return np.argmin(P, axis=1)
+
+
+MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "model", "cost_model.npz")
+
+
+def save_model(m, path=MODEL_PATH, meta=None):
+    """Weights, input normalisation and output offset of a trained model."""
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    params = m.mlp.get_params()
+    np.savez(path, mu=m.mu, sd=m.sd, ymu=np.float64(m.ymu), hidden=np.int64(m.mlp.H),
+             n_out=np.int64(m.mlp.n_out), meta=json.dumps(meta or {}),
+             **{"p%d" % i: p for i, p in enumerate(params)})
+
+
+def load_model(ctx, path=MODEL_PATH):
+    """The committed cost model on this context (raises if there is none)."""
+    z = np.load(path, allow_pickle=False)
+    m = CostModel(ctx, int(z["n_out"]), hidden=int(z["hidden"]))
+    m.mlp.set_params([z["p%d" % i] for i in range(6)])
+    m.mu, m.sd, m.ymu = z["mu"].astype(np.float32), z["sd"].astype(np.float32), float(z["ymu"])
+    m.trained = True
+    m.n_out = int(z["n_out"])
+    return m
 
 
 def regret(Y, V, choice):
