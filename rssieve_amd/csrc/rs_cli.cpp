@@ -16,6 +16,7 @@
 
 #include "rssieve.h"
 #include "rssieve_io.h"
+#include "rssieve_mlp.h"
 
 namespace {
 
@@ -412,6 +413,28 @@ int sieve(Options o) {
   if (rs_job_create(r.ctx, base, kcount, ks.data(), cs.data(), nmin, nmax, rs_abcd_bitmap(r.abcd), wps, &prm,
                     &r.job))
     die_lib("rs_job_create");
+  if (o.merged && st.pmax - st.p > 4096) {
+    // -t: time every candidate configuration on a short sample at the start of
+    // the range (the job's bitmap and counters are restored afterwards)
+    std::vector<rs_params> cand(64);
+    int nc = 0;
+    if (rs_tune_candidates(r.ctx, nmin, nmax, kcount, cand.data(), (int)cand.size(), &nc)) die_lib("rs_tune_candidates");
+    nc = std::min<int>(nc, (int)cand.size());
+    for (int i = 0; i < nc; i++) {
+      cand[i].k31_compat = prm.k31_compat;
+      cand[i].factor_cap = prm.factor_cap;
+    }
+    // about 2^16 primes: width ~ 2^16 * ln p
+    const double lnp = std::log((double)st.p);
+    uint64_t width = (uint64_t)(65536.0 * lnp);
+    if (width > st.pmax - st.p) width = st.pmax - st.p;
+    std::vector<double> secs(nc);
+    std::vector<uint64_t> pr(nc);
+    int best = 0;
+    if (rs_autotune(r.job, st.p, width, cand.data(), nc, secs.data(), pr.data(), &best)) die_lib("rs_autotune");
+    prm = cand[best];
+    printf("Autotuned: candidate %d of %d (%s)\n", best, nc, prm.algo == RS_ALGO_MERGED ? "merged table" : "three lists");
+  }
   uint32_t M, QQ, MM;
   rs_params_derive(&prm, nmin, nmax, &M, &QQ, &MM);
   printf("Q:%u M:%u QQ:%u MM:%u HT:%u LS:%u\n", prm.Q, M, QQ, MM, prm.hsize, prm.threads);

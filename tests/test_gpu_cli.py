@@ -82,3 +82,27 @@ def test_cli_selftest(tmp_path):
     r = _run(["-s"], str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
     assert "1000133 | 91549*2^54378870+1 found" in r.stdout
+
+
+def test_cli_tuned_same_terms(tmp_path):
+    """-t (autotuned merged table) eliminates exactly the terms the default
+    reference split does."""
+    base, ks, cs = 2, [3, 5, 7, 11, 13, 15], [1, -1, 1, -1, 1, -1]
+    _make_abcd(tmp_path / "in.abcd", base, ks, cs, 40000, 90000, 0.5, 7)
+    outs = []
+    for flags, name in [([], "plain"), (["-t"], "tuned")]:
+        d = tmp_path / name
+        d.mkdir()
+        (d / "in.abcd").write_text((tmp_path / "in.abcd").read_text())
+        r = _run(["-p", "5e8", "-P", "503000000", "-i", "in.abcd", "-f", "out.txt"] + flags, str(d), timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        if flags:
+            assert "Autotuned: candidate" in r.stdout
+        terms = set()
+        for l in _factor_lines(d / "out.txt"):
+            if l == "no factors":
+                continue
+            p, k, s, n = R.parse_factor(l, 2)
+            terms.add((k, s, n))
+        outs.append(terms)
+    assert outs[0] == outs[1]

@@ -1,0 +1,41 @@
+"""What the learned cost model gains over the configuration rule without it.
+
+Data: tests/golden/tune_records.npz, autotuner timings of synthetic jobs
+(written by tests/golden/make_tune_records.py).  The MLP (5 -> 64 -> 64 ->
+#candidates, fp32, the library's HIP kernels) is trained on the first rows and
+judged on the last 256: its regret (time of its pick over the measured best)
+against the reference split's (build_giant_kernels, candidate 0), the rule a
+job gets without the model."""
+import os
+
+import numpy as np
+import pytest
+
+import rssieve_amd as R
+from rssieve_amd import tuning
+
+pytestmark = pytest.mark.gpu
+
+RECORDS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tune_records.npz")
+
+
+@pytest.mark.skipif(not os.path.isfile(RECORDS), reason="no recorded autotuner timings")
+def test_cost_model_beats_reference_rule(ctx):
+    z = np.load(RECORDS, allow_pickle=False)
+    X, Y, V = z["X"], z["Y"], z["V"]
+    n_test = 256
+    ntr = len(X) - n_test
+    assert ntr >= 256
+    m = tuning.CostModel(ctx, Y.shape[1], seed=5)
+    m.fit(X[:ntr], Y[:ntr], V[:ntr])
+    choice = m.choose(X[ntr:], V[ntr:])
+    r_model = tuning.regret(Y[ntr:], V[ntr:], choice)
+    r_ref = tuning.regret(Y[ntr:], V[ntr:], np.zeros(n_test, np.int64))
+    gain = float(np.exp(np.mean(np.log((1 + r_ref) / (1 + r_model)))))
+    print("held-out regret: model mean %.3f median %.3f | reference rule mean %.3f median %.3f | geomean gain %.2fx"
+          % (r_model.mean(), np.median(r_model), r_ref.mean(), np.median(r_ref), gain), flush=True)
+    assert r_model.mean() < r_ref.mean()
+    assert gain > 1.1
+    # the untrained model falls back to the reference rule
+    fresh = tuning.CostModel(ctx, Y.shape[1])
+    assert np.all(fresh.choose(X[ntr:]) == 0)
