@@ -17,7 +17,7 @@
 
 namespace rs {
 hipError_t bsgs_blocks_per_cu(uint32_t threads, uint32_t hsize, int* out);
-size_t bsgs_lds_bytes(uint32_t hsize);
+size_t bsgs_lds_bytes(uint32_t hsize, uint32_t threads);
 hipError_t launch_set_cursor(SegJob* sj, uint64_t pmin, uint64_t pmax, uint32_t threads, uint64_t range,
                              uint32_t gen_blocks, hipStream_t s);
 hipError_t launch_round(SegJob* jobs, BatchState* bs, uint32_t njobs, uint32_t gen_blocks, uint32_t setup_blocks,
@@ -250,9 +250,9 @@ static int check_params(const rs_ctx* ctx, const rs_params* prm, const char* who
   if (prm->Q < 16 || entries > 16384 || !pow2(prm->hsize) || prm->hsize < 64 || prm->hsize <= entries)
     return fail(RS_E_ARG, "%s: need at most 16384 table entries and a power-of-two hsize >= 64 above them "
                 "(Q=%u entries=%u hsize=%u)", who, prm->Q, entries, prm->hsize);
-  if (bsgs_lds_bytes(prm->hsize) + 4096 > ctx->lds_per_block)
+  if (bsgs_lds_bytes(prm->hsize, prm->threads) + 4096 > ctx->lds_per_block)
     return fail(RS_E_ARG, "%s: hash of %u slots needs %zu LDS bytes, device has %u", who, prm->hsize,
-                bsgs_lds_bytes(prm->hsize), ctx->lds_per_block);
+                bsgs_lds_bytes(prm->hsize, prm->threads), ctx->lds_per_block);
   return RS_OK;
 }
 
@@ -492,7 +492,7 @@ extern "C" int rs_sieve_batch(rs_job** jobs, int njobs, const uint64_t* pmin, co
     cap_max = std::max(cap_max, j->seg_cap);
     R = std::max(R, rounds_for(pmin[i], pmax[i], range));
   }
-  if (bsgs_lds_bytes(hsize) + 4096 > ctx->lds_per_block) return fail(RS_E_ARG, "rs_sieve_batch: table too large");
+  if (bsgs_lds_bytes(hsize, T) + 4096 > ctx->lds_per_block) return fail(RS_E_ARG, "rs_sieve_batch: table too large");
   int bpc = 0;
   if (bsgs_blocks_per_cu(T, hsize, &bpc) != hipSuccess || bpc < 1) bpc = 1;
   const uint32_t bsgs_blocks = (uint32_t)bpc * ctx->cus;

@@ -245,6 +245,24 @@ __device__ __forceinline__ void powmod_lg2(uint64_t x0, uint64_t x1, uint64_t ex
   *r1 = a1;
 }
 
+// powmod_lg2 on plain residues in binary64
+__device__ __forceinline__ void fp_powmod_lg2(double x0, double x1, uint64_t exp, double p, double pinv, double* r0,
+                                              double* r1) {
+  uint64_t bit = 0x8000000000000000ull >> ((clz64(exp) + 1) & 63);
+  double a0 = x0, a1 = x1;
+  while (bit) {
+    a0 = fp_mulmod(a0, a0, p, pinv);
+    a1 = fp_mulmod(a1, a1, p, pinv);
+    if (exp & bit) {
+      a0 = fp_mulmod(a0, x0, p, pinv);
+      a1 = fp_mulmod(a1, x1, p, pinv);
+    }
+    bit >>= 1;
+  }
+  *r0 = a0;
+  *r1 = a1;
+}
+
 // d of the power-residue tests (setup.cl:22-48)
 __constant__ uint32_t g_prd[12] = {3, 4, 5, 7, 8, 9, 11, 13, 17, 19, 23, 29};
 
@@ -324,6 +342,13 @@ __global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
   //  lane 0: Fermat inverse total^(p-2)          (setup.cl:154)
   //  lanes 1..12: base^((p-1)/d) for d | p-1       (setup.cl:22-48)
   //  lane 13: base^((p-1)/2)                       (setup.cl:51-53)
+  // Odd p below 2^51 exponentiate in binary64 on plain residues (x*R^-1,
+  // one Montgomery product away) and return to Montgomery form through R^2.
+  const bool fp = (p & 1) && p < kFpLimit;
+  const double pd = (double)p, pinv = fp ? 1.0 / pd : 0.0;
+  auto to_plain = [&](uint64_t m) { return (double)mmul(m, 1, p, q); };
+  auto to_mont = [&](double x) { return mmul(fp_to_u64(x), r2, p, q); };
+
   const uint64_t pm = p - 1;
   uint64_t ex = 0, x = mbase;
   bool dvalid = false;
@@ -339,7 +364,7 @@ __global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
     ex = pm >> 1;
   }
   uint64_t r = one;
-  if (ex) r = powmod_lg(x, ex, p, q);
+  if (ex) r = fp ? to_mont(fp_powmod_lg(to_plain(x), ex, pd, pinv)) : powmod_lg(x, ex, p, q);
   const uint64_t inv_total = shfl64(r, 0);
   const bool invertible = mmul(total, inv_total, p, q) == one;  // setup.cl:156-162
   const uint64_t dmask = __ballot(dvalid && r == one) >> 1;      // passing d tests, bit i = g_prd[i]
@@ -365,27 +390,41 @@ __global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
   const uint64_t h1 = (act1 && job->cpos[lane + 64]) ? mmul(nmo, kinv1, p, q) : kinv1;
 
   // prefilter (setup.cl:59-87); the order of the tests does not change the result
+  // h^e for both slots, as +1 (one), -1 (p - 1) or 0 (anything else)
+  const double hd0 = fp ? to_plain(h0) : 0.0, hd1 = fp ? to_plain(h1) : 0.0;
+  auto pow_sign2 = [&](uint64_t e, int* s0, int* s1) {
+    if (fp) {
+      double a0, a1;
+      fp_powmod_lg2(hd0, hd1, e, pd, pinv, &a0, &a1);
+      *s0 = (a0 == 1.0) - (a0 == pd - 1.0);
+      *s1 = (a1 == 1.0) - (a1 == pd - 1.0);
+    } else {
+      uint64_t a0, a1;
+      powmod_lg2(h0, h1, e, p, q, &a0, &a1);
+      *s0 = (a0 == one) - (a0 == nmo);
+      *s1 = (a1 == one) - (a1 == nmo);
+    }
+  };
   int cls0 = act0 ? 1 : 0, cls1 = act1 ? 1 : 0;
   uint64_t dm = dmask;
   while (dm) {
     const int i = __ffsll((long long)dm) - 1;
     dm &= dm - 1;
-    const uint64_t e = shfl64(ex, i + 1);
-    uint64_t ra0, ra1;
-    powmod_lg2(h0, h1, e, p, q, &ra0, &ra1);
-    if (ra0 != one) cls0 = 0;
-    if (ra1 != one) cls1 = 0;
+    int s0, s1;
+    pow_sign2(shfl64(ex, i + 1), &s0, &s1);
+    if (s0 != 1) cls0 = 0;
+    if (s1 != 1) cls1 = 0;
   }
   {
-    uint64_t ra0, ra1;
-    powmod_lg2(h0, h1, pm >> 1, p, q, &ra0, &ra1);
+    int s0, s1;
+    pow_sign2(pm >> 1, &s0, &s1);
     if (cls0) {
-      if (resg == -1) cls0 = (ra0 == one) ? 2 : (ra0 == nmo) ? 3 : 0;
-      else if (resg == 1 && ra0 == nmo) cls0 = 0;
+      if (resg == -1) cls0 = (s0 == 1) ? 2 : (s0 == -1) ? 3 : 0;
+      else if (resg == 1 && s0 == -1) cls0 = 0;
     }
     if (cls1) {
-      if (resg == -1) cls1 = (ra1 == one) ? 2 : (ra1 == nmo) ? 3 : 0;
-      else if (resg == 1 && ra1 == nmo) cls1 = 0;
+      if (resg == -1) cls1 = (s1 == 1) ? 2 : (s1 == -1) ? 3 : 0;
+      else if (resg == 1 && s1 == -1) cls1 = 0;
     }
   }
 
@@ -407,7 +446,8 @@ __global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
   else if (lane == 3) ye = T;
   else if (lane == 4) ye = 2 * T;
   else if (lane == 5) ye = job->Qref - 1;
-  if (lane == 0 || (lane >= 3 && lane <= 5)) y = powmod_sm(yb, ye, p, q, one);
+  if (lane == 0 || (lane >= 3 && lane <= 5))
+    y = fp ? to_mont(fp_powmod_sm(to_plain(yb), ye, pd, pinv)) : powmod_sm(yb, ye, p, q, one);
   const uint64_t bn = shfl64(y, 0), bT0 = shfl64(y, 3), bT1 = shfl64(y, 4);
   const uint64_t binv = mmul(shfl64(y, 5), gQref_inv, p, q);
   ye = 0;
@@ -415,7 +455,7 @@ __global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
   else if (lane == 1) ye = job->Q * T;
   else if (lane == 2) ye = 2 * job->Q * T;
   y = one;
-  if (lane < 3) y = powmod_sm(binv, ye, p, q, one);
+  if (lane < 3) y = fp ? to_mont(fp_powmod_sm(to_plain(binv), ye, pd, pinv)) : powmod_sm(binv, ye, p, q, one);
   const uint64_t gQ_inv = shfl64(y, 0), gT0 = shfl64(y, 1), gT1 = shfl64(y, 2);
   const uint64_t g1 = mmul(gQ_inv, gQ_inv, p, q);
 
@@ -567,12 +607,14 @@ struct TabCtx {
   uint32_t base, nstart, stride;
 };
 
-// Entry e whose baby value b^(nstart + stride*e) equals g, or -1.
-__device__ __noinline__ int tab_find(const TabCtx& t, uint64_t g) {
+// Entry e whose baby value b^(nstart + stride*e) equals g, or -1.  Out of
+// line (rarely taken) with scalar arguments, so nothing goes through scratch.
+__device__ __noinline__ int tab_find_at(const uint32_t* tab, uint32_t bmask, uint64_t mbase, uint64_t p, uint64_t q,
+                                        uint64_t one, uint32_t base, uint32_t nstart, uint32_t stride, uint64_t g) {
   const uint32_t fp = tab_fp(g);
-  uint32_t b = (uint32_t)g & t.bmask;
+  uint32_t b = (uint32_t)g & bmask;
   while (true) {
-    const uint4* bk = (const uint4*)(t.tab + b * kBucket);
+    const uint4* bk = (const uint4*)(tab + b * kBucket);
     const uint4 x = bk[0], y = bk[1];
     const uint32_t w[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
     bool full = true;
@@ -581,33 +623,36 @@ __device__ __noinline__ int tab_find(const TabCtx& t, uint64_t g) {
       if (w[s] == 0u) full = false;
       if ((w[s] >> 14) == fp) {
         const uint32_t e = w[s] & 0x3FFFu;
-        if (basepow_sm(t.mbase, t.nstart + t.stride * e, t.base, t.p, t.q, t.one) == g) return (int)e;
+        if (basepow_sm(mbase, nstart + stride * e, base, p, q, one) == g) return (int)e;
       }
     }
     if (!full) return -1;
-    b = (b + 1) & t.bmask;
+    b = (b + 1) & bmask;
   }
+}
+__device__ __forceinline__ int tab_find(const TabCtx& t, uint64_t g) {
+  return tab_find_at(t.tab, t.bmask, t.mbase, t.p, t.q, t.one, t.base, t.nstart, t.stride, g);
 }
 
 // Blocked Bloom filter in front of the table: one 32-bit word per table slot,
 // two bits per baby value, both in the word chosen by the residue's low bits.
 // With at most hsize/2 values the false-positive rate is below 2^-10, so a
 // giant step costs one ds_read_b32 and a few integer ops; only the rare
-// positives consult the table.
-__device__ __forceinline__ uint32_t flt_key(uint64_t v) {
-  return (uint32_t)v ^ ((uint32_t)(v >> 32) * 0x9E3779B1u);
-}
-__device__ __forceinline__ uint32_t flt_bits(uint32_t x) {
-  return (1u << ((x >> 16) & 31u)) | (1u << ((x >> 21) & 31u));
-}
-__device__ __forceinline__ void flt_insert(uint32_t* flt, uint32_t fmask, uint64_t v) {
-  const uint32_t x = flt_key(v);
+// positives consult the table.  The key is the low 32 bits of the residue (a
+// residue of a product mod p is spread evenly over them): bits 0..13 pick the
+// word (hsize <= 16384), bits 16..20 and 24..28 the two bits (byte-aligned
+// fields, so each bit is one shift with a sub-dword operand select).
+__device__ __forceinline__ uint32_t flt_bits(uint32_t x) { return (1u << ((x >> 16) & 31u)) | (1u << ((x >> 24) & 31u)); }
+__device__ __forceinline__ void flt_insert(uint32_t* flt, uint32_t fmask, uint32_t x) {
   atomicOr(&flt[x & fmask], flt_bits(x));
 }
-__device__ __forceinline__ bool flt_maybe(const uint32_t* flt, uint32_t fmask, uint64_t v) {
-  const uint32_t x = flt_key(v);
+__device__ __forceinline__ bool flt_maybe(const uint32_t* flt, uint32_t fmask, uint32_t x) {
   const uint32_t m = flt_bits(x);
   return (flt[x & fmask] & m) == m;
+}
+// the filter bits of x that are clear: 0 iff x may be in the table
+__device__ __forceinline__ uint32_t flt_miss(const uint32_t* flt, uint32_t fmask, uint32_t x) {
+  return ~flt[x & fmask] & flt_bits(x);
 }
 
 // One persistent kernel serves every job of a batch.  An item is (job, prime,
@@ -626,14 +671,22 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
   const uint32_t njobs = bs->njobs;
   const uint32_t nitems = bs->item_base[njobs];
 
+  // Thread 0 takes items from the queue one ahead: the atomic for the next
+  // item is in flight while this one runs.  A block's items only grow, so
+  // its job index only moves forward.
+  uint32_t nxt = 0, jcur = 0, jend = 0;  // jend: first item past job jcur
+  if (tid == 0) {
+    nxt = atomicAdd(&bs->work_next, 1u);
+    jend = bs->item_base[1];
+  }
   for (;;) {
     if (tid == 0) {
-      const uint32_t it = atomicAdd(&bs->work_next, 1u);
-      uint32_t j = 0;
+      const uint32_t it = nxt;
+      nxt = atomicAdd(&bs->work_next, 1u);
       if (it < nitems)
-        while (it >= bs->item_base[j + 1]) j++;
+        while (it >= jend) jend = bs->item_base[++jcur + 1];
       s_item = it;
-      s_job = j;
+      s_job = jcur;
       s_dups = 0;
       s_claims = 0;
     }
@@ -651,6 +704,7 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
     if (nk == 0) continue;
     const uint32_t hsize = job->hsize, bmask = (hsize / kBucket) - 1, fmask = hsize - 1;
     uint32_t* flt = tab + hsize;
+    uint64_t* s_g = (uint64_t*)(tab + 2 * hsize);  // T giant-step powers, after table and filter
     const uint32_t base = job->base;
     const int K = job->kcount;
     const uint32_t nmin = job->nmin, nmax = job->nmax;
@@ -680,15 +734,33 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
     }
     if (tid == 0) s_order = 0xFFFFFFFFu;
     __syncthreads();
-    // baby steps: entry e = tid + T*i holds b^(nstart + stride*e)
-    const uint64_t vstart = mmul(v0, basepow_sm(mbase, stride * tid, base, p, q, one), p, q);
-    {
+    // baby steps: entry e = tid + T*i holds b^(nstart + stride*e).  Below
+    // 2^51 the steps multiply in binary64 by the plain residue of the stride,
+    // which keeps the Montgomery representation of the integer path.
+    const bool fp = (p & 1) && p < kFpLimit;
+    const double pd = (double)p, pinv = fp ? 1.0 / pd : 0.0;
+    const double bd = fp ? (double)mmul(mbase, 1, p, q) : 0.0;  // plain b
+    const uint64_t vstart = fp ? fp_to_u64(fp_mulmod((double)v0, fp_powmod_sm(bd, stride * tid, pd, pinv), pd, pinv))
+                               : mmul(v0, basepow_sm(mbase, stride * tid, base, p, q, one), p, q);
+    if (fp) {
+      const double bsd = (double)mmul(bstep, 1, p, q);
+      const double v0d = (double)v0;
+      double v = (double)vstart;
+      for (uint32_t e = tid; e < nent; e += T) {
+        const uint32_t j = e * stride;
+        const uint64_t vi = fp_to_u64(v);
+        if (j && v == v0d) atomicMin(&s_order, j);
+        tab_insert(tab, bmask, vi, e);
+        if ((uint32_t)vi) flt_insert(flt, fmask, (uint32_t)vi);
+        v = fp_mulmod(v, bsd, pd, pinv);
+      }
+    } else {
       uint64_t v = vstart;
       for (uint32_t e = tid; e < nent; e += T) {
         const uint32_t j = e * stride;
         if (j && v == v0) atomicMin(&s_order, j);
         tab_insert(tab, bmask, v, e);
-        if ((uint32_t)v) flt_insert(flt, fmask, v);
+        if ((uint32_t)v) flt_insert(flt, fmask, (uint32_t)v);
         v = mmul(v, bstep, p, q);
       }
     }
@@ -720,28 +792,106 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
       }
       work = order / stride + nk;
       probes = nk;
+    } else if (fp) {
+      // giant steps in binary64: the targets (Montgomery residues) times the
+      // plain residue of G^q give the Montgomery residues the table holds;
+      // fp_mulmod_bits leaves kFpBias + residue, whose low word is the key
+      const double pc = pd + kFpTwo52;
+      double gq = fp_powmod_sm((double)mmul(R->g[cp], 1, p, q), tid, pd, pinv);
+      const double gtd = (double)mmul(R->gT[cp], 1, p, q);
+      for (uint32_t i = tid; i < nk; i += T) s_h[i] = (uint64_t)__double_as_longlong((double)s_h[i]);
+      __syncthreads();
+      const uint32_t F = Ml / T;
+      for (uint32_t r = 0; r < F; r++) {
+        const uint32_t nq = nstart + (tid + r * T) * Ql;
+        uint32_t i = 0;
+        for (; i + 4 <= nk; i += 4) {
+          const uint64_t u0 = fp_mulmod_bits(__longlong_as_double(s_h[i]), gq, pd, pinv, pc);
+          const uint64_t u1 = fp_mulmod_bits(__longlong_as_double(s_h[i + 1]), gq, pd, pinv, pc);
+          const uint64_t u2 = fp_mulmod_bits(__longlong_as_double(s_h[i + 2]), gq, pd, pinv, pc);
+          const uint64_t u3 = fp_mulmod_bits(__longlong_as_double(s_h[i + 3]), gq, pd, pinv, pc);
+          const uint32_t x0 = flt_miss(flt, fmask, (uint32_t)u0), x1 = flt_miss(flt, fmask, (uint32_t)u1);
+          const uint32_t x2 = flt_miss(flt, fmask, (uint32_t)u2), x3 = flt_miss(flt, fmask, (uint32_t)u3);
+          if (min(min(x0, x1), min(x2, x3)) == 0u) {
+            const uint64_t gg[4] = {u0 ^ kFpBias, u1 ^ kFpBias, u2 ^ kFpBias, u3 ^ kFpBias};
+            const uint32_t xx[4] = {x0, x1, x2, x3};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              if (xx[u]) continue;
+              const int e = tab_find(tc, gg[u]);
+              if (e >= 0) {
+                const uint32_t kk = s_k[i + u];
+                claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu, nq + stride * (uint32_t)e + (kk >> 31), &s_dups,
+                      &s_claims);
+              }
+            }
+          }
+        }
+        for (; i < nk; i++) {
+          const uint64_t u0 = fp_mulmod_bits(__longlong_as_double(s_h[i]), gq, pd, pinv, pc);
+          if (flt_maybe(flt, fmask, (uint32_t)u0)) {
+            const int e = tab_find(tc, u0 ^ kFpBias);
+            if (e >= 0) {
+              const uint32_t kk = s_k[i];
+              claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu, nq + stride * (uint32_t)e + (kk >> 31), &s_dups,
+                    &s_claims);
+            }
+          }
+        }
+        gq = fp_mulmod(gq, gtd, pd, pinv);
+      }
+      // The last m = Ml - F*T giant steps: their m*nk probes are spread over
+      // every thread, (giant step j, target i) for L = tid, tid + T, ...
+      // with j = L mod m and i = L / m; thread j < m holds G^(F*T + j).
+      const uint32_t m = Ml - F * T;
+      if (m) {
+        if (tid < m) s_g[tid] = (uint64_t)__double_as_longlong(gq);
+        __syncthreads();
+        const uint32_t da = T / m, db = T % m;
+        uint32_t j = tid % m, i = tid / m;
+        for (uint32_t L = tid; L < m * nk; L += T) {
+          const uint64_t u0 = fp_mulmod_bits(__longlong_as_double(s_h[i]), __longlong_as_double(s_g[j]), pd, pinv, pc);
+          if (flt_maybe(flt, fmask, (uint32_t)u0)) {
+            const int e = tab_find(tc, u0 ^ kFpBias);
+            if (e >= 0) {
+              const uint32_t kk = s_k[i];
+              claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu,
+                    nstart + (F * T + j) * Ql + stride * (uint32_t)e + (kk >> 31), &s_dups, &s_claims);
+            }
+          }
+          i += da;
+          j += db;
+          if (j >= m) {
+            j -= m;
+            i++;
+          }
+        }
+      }
+      work = (unsigned long long)nent + (unsigned long long)Ml * (nk + 1);
+      probes = (unsigned long long)Ml * nk;
     } else {
       // giant steps: thread tid handles q = tid, tid + T, ...; the targets
       // are taken four at a time so four products and four filter reads are
       // in flight together
       const uint64_t g = R->g[cp], gT = R->gT[cp];
       uint64_t Gq = powmod_sm(g, tid, p, q, one);
-      for (uint32_t qq = tid; qq < Ml; qq += T) {
-        const uint32_t nq = nstart + qq * Ql;
+      const uint32_t F = Ml / T;
+      for (uint32_t r = 0; r < F; r++) {
+        const uint32_t nq = nstart + (tid + r * T) * Ql;
         uint32_t i = 0;
         for (; i + 4 <= nk; i += 4) {
           const uint64_t g0 = mmul(s_h[i], Gq, p, q);
           const uint64_t g1 = mmul(s_h[i + 1], Gq, p, q);
           const uint64_t g2 = mmul(s_h[i + 2], Gq, p, q);
           const uint64_t g3 = mmul(s_h[i + 3], Gq, p, q);
-          const bool m0 = flt_maybe(flt, fmask, g0), m1 = flt_maybe(flt, fmask, g1);
-          const bool m2 = flt_maybe(flt, fmask, g2), m3 = flt_maybe(flt, fmask, g3);
-          if (m0 | m1 | m2 | m3) {
+          const uint32_t x0 = flt_miss(flt, fmask, (uint32_t)g0), x1 = flt_miss(flt, fmask, (uint32_t)g1);
+          const uint32_t x2 = flt_miss(flt, fmask, (uint32_t)g2), x3 = flt_miss(flt, fmask, (uint32_t)g3);
+          if (min(min(x0, x1), min(x2, x3)) == 0u) {
             const uint64_t gg[4] = {g0, g1, g2, g3};
-            const bool mm[4] = {m0, m1, m2, m3};
+            const uint32_t xx[4] = {x0, x1, x2, x3};
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-              if (!mm[u]) continue;
+              if (xx[u]) continue;
               const int e = tab_find(tc, gg[u]);
               if (e >= 0) {
                 const uint32_t kk = s_k[i + u];
@@ -753,7 +903,7 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
         }
         for (; i < nk; i++) {
           const uint64_t g0 = mmul(s_h[i], Gq, p, q);
-          if (flt_maybe(flt, fmask, g0)) {
+          if (flt_maybe(flt, fmask, (uint32_t)g0)) {
             const int e = tab_find(tc, g0);
             if (e >= 0) {
               const uint32_t kk = s_k[i];
@@ -763,6 +913,30 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
           }
         }
         Gq = mmul(Gq, gT, p, q);
+      }
+      const uint32_t m = Ml - F * T;  // the last giant steps, as in the binary64 branch
+      if (m) {
+        if (tid < m) s_g[tid] = Gq;
+        __syncthreads();
+        const uint32_t da = T / m, db = T % m;
+        uint32_t j = tid % m, i = tid / m;
+        for (uint32_t L = tid; L < m * nk; L += T) {
+          const uint64_t g0 = mmul(s_h[i], s_g[j], p, q);
+          if (flt_maybe(flt, fmask, (uint32_t)g0)) {
+            const int e = tab_find(tc, g0);
+            if (e >= 0) {
+              const uint32_t kk = s_k[i];
+              claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu,
+                    nstart + (F * T + j) * Ql + stride * (uint32_t)e + (kk >> 31), &s_dups, &s_claims);
+            }
+          }
+          i += da;
+          j += db;
+          if (j >= m) {
+            j -= m;
+            i++;
+          }
+        }
       }
       work = (unsigned long long)nent + (unsigned long long)Ml * (nk + 1);
       probes = (unsigned long long)Ml * nk;
@@ -795,10 +969,11 @@ __global__ void k_round_items(SegJob* __restrict__ jobs, BatchState* __restrict_
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-size_t bsgs_lds_bytes(uint32_t hsize) { return (size_t)hsize * 8; }  // table + filter
+// table (hsize u32) + filter (hsize u32) + one u64 power per thread
+size_t bsgs_lds_bytes(uint32_t hsize, uint32_t threads) { return (size_t)hsize * 8 + (size_t)threads * 8; }
 
 hipError_t bsgs_blocks_per_cu(uint32_t threads, uint32_t hsize, int* out) {
-  const size_t lds = bsgs_lds_bytes(hsize);
+  const size_t lds = bsgs_lds_bytes(hsize, threads);
   switch (threads) {
     case 128: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_bsgs<128>, 128, lds);
     case 256: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_bsgs<256>, 256, lds);
@@ -828,7 +1003,7 @@ hipError_t launch_round_prep(SegJob* jobs, BatchState* bs, uint32_t njobs, uint3
 
 hipError_t launch_round_bsgs(SegJob* jobs, BatchState* bs, uint32_t threads, uint32_t hsize, uint32_t bsgs_blocks,
                              hipStream_t s) {
-  const size_t lds = bsgs_lds_bytes(hsize);
+  const size_t lds = bsgs_lds_bytes(hsize, threads);
   switch (threads) {
     case 128: hipLaunchKernelGGL(k_bsgs<128>, dim3(bsgs_blocks), dim3(128), lds, s, jobs, bs); break;
     case 256: hipLaunchKernelGGL(k_bsgs<256>, dim3(bsgs_blocks), dim3(256), lds, s, jobs, bs); break;

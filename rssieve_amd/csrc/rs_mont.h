@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #define RS_HD __host__ __device__ __forceinline__
 #else
+#include <math.h>
+#include <string.h>
 #define RS_HD static inline
 #endif
 
@@ -133,6 +135,103 @@ RS_HD uint64_t basepow_sm(uint64_t mbase, uint32_t exp, uint32_t base, uint64_t 
   while (bit) {
     a = mmul(a, a, p, q);
     if (exp & bit) a = mul_base(a, mbase, base, p, q);
+    bit >>= 1;
+  }
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+// Exact modular products in binary64 for odd p < 2^51 (full-rate FP64 FMA on
+// gfx950, against quarter/half-rate 32x32->64 integer products).
+//
+// For integers a, b in [0, p):
+//   h = fl(a*b), l = fma(a, b, -h)   h + l = a*b exactly (the rounding error
+//                                     of a product is a double, TwoProduct)
+//   s = fma(h, pinv, 1.5*2^52)       one rounding of h*pinv + 1.5*2^52 to the
+//   qf = s - 1.5*2^52                 integer grid: qf = rint(h*pinv), exactly
+//   r = fma(-qf, p, h)               exact: |h - qf*p| < 2^53 (below)
+//   t = r + l                        exact: t = a*b - qf*p
+// with pinv = fl(1/p).  Error of the quotient: h*pinv = (ab/p)(1+e1)(1+e2),
+// |e1|,|e2| <= 2^-53 and ab/p < p, so |h*pinv - ab/p| <= p*(2^-52 + 2^-106),
+// which is below 1/2 for p <= 2^51 - 1; rint adds at most 1/2, so
+// |ab/p - qf| < 1.  Hence t = ab - qf*p lies in (-p, p), |r| <= |l| + |t| <
+// 2^49 + 2^51, and one conditional add of p gives the canonical residue.
+// Adding 2^52 to a canonical value c < 2^51 leaves the bits
+// 0x4330000000000000 + c, so the integer residue costs one add.
+//
+// Montgomery residues multiplied by plain residues stay Montgomery residues
+// (aR * b = (ab)R mod p), which lets the sieve keep its tables in the
+// integer kernels' representation while stepping in binary64.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kFpLimit = 1ull << 51;
+constexpr double kFpTwo52 = 4503599627370496.0;   // 2^52
+constexpr double kFpRound = 6755399441055744.0;   // 1.5 * 2^52
+constexpr uint64_t kFpBias = 0x4330000000000000ull;  // bits of 2^52
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define RS_FMUL(a, b) __dmul_rn(a, b)
+#define RS_FADD(a, b) __dadd_rn(a, b)
+#define RS_FMA(a, b, c) __fma_rn(a, b, c)
+#define RS_AS_U64(x) ((uint64_t)__double_as_longlong(x))
+#else
+RS_HD uint64_t rs_as_u64_host(double x) {
+  uint64_t u;
+  memcpy(&u, &x, 8);
+  return u;
+}
+#define RS_FMUL(a, b) ((double)(a) * (double)(b))
+#define RS_FADD(a, b) ((double)(a) + (double)(b))
+#define RS_FMA(a, b, c) fma(a, b, c)
+#define RS_AS_U64(x) rs_as_u64_host(x)
+#endif
+
+// a*b - qf*p in (-p, p) (the derivation above)
+RS_HD double fp_mulmod_signed(double a, double b, double p, double pinv) {
+  const double h = RS_FMUL(a, b);
+  const double l = RS_FMA(a, b, -h);
+  const double qf = RS_FADD(RS_FMA(h, pinv, kFpRound), -kFpRound);
+  const double r = RS_FMA(-qf, p, h);
+  return RS_FADD(r, l);
+}
+
+// a*b mod p, canonical, as a double
+RS_HD double fp_mulmod(double a, double b, double p, double pinv) {
+  const double t = fp_mulmod_signed(a, b, p, pinv);
+  return t < 0.0 ? RS_FADD(t, p) : t;
+}
+
+// kFpBias + (a*b mod p): the integer residue plus a constant in the exponent
+// field; pc = p + 2^52 (exact for p < 2^51)
+RS_HD uint64_t fp_mulmod_bits(double a, double b, double p, double pinv, double pc) {
+  const double t = fp_mulmod_signed(a, b, p, pinv);
+  return RS_AS_U64(RS_FADD(t, t < 0.0 ? pc : kFpTwo52));
+}
+
+// integer residue of a canonical double
+RS_HD uint64_t fp_to_u64(double c) { return RS_AS_U64(RS_FADD(c, kFpTwo52)) ^ kFpBias; }
+
+// x^exp on plain residues in binary64 with the control flow of powmod_lg
+// (the same masked shift, so the same effective exponent)
+RS_HD double fp_powmod_lg(double x, uint64_t exp, double p, double pinv) {
+  uint64_t bit = 0x8000000000000000ull >> ((clz64(exp) + 1) & 63);
+  double a = x;
+  while (bit) {
+    a = fp_mulmod(a, a, p, pinv);
+    if (exp & bit) a = fp_mulmod(a, x, p, pinv);
+    bit >>= 1;
+  }
+  return a;
+}
+
+// x^exp on plain residues in binary64 with the control flow of powmod_sm
+RS_HD double fp_powmod_sm(double x, uint32_t exp, double p, double pinv) {
+  if (!exp) return 1.0;
+  if (exp == 1) return x;
+  uint32_t bit = 0x80000000u >> ((clz32(exp) + 1) & 31);
+  double a = x;
+  while (bit) {
+    a = fp_mulmod(a, a, p, pinv);
+    if (exp & bit) a = fp_mulmod(a, x, p, pinv);
     bit >>= 1;
   }
   return a;

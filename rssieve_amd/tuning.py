@@ -179,6 +179,15 @@ def load_model(ctx, path=MODEL_PATH):
     return m
 
 
+def best_fixed(Y, V):
+    """The non-learned rule fitted to the same records: the one candidate with
+    the lowest mean log time over the rows where it ran."""
+    L = np.where(V, np.log(np.where(V, Y, 1.0)), np.nan)
+    score = np.nanmean(L, axis=0)
+    score = np.where(V.mean(0) > 0.99, score, np.inf)  # valid for (almost) every job
+    return int(np.argmin(score))
+
+
 def regret(Y, V, choice):
     """time(chosen) / time(best measured) - 1, per row."""
     Yv = np.where(V, Y, np.inf)
@@ -215,7 +224,10 @@ def main(argv=None):
         ch = m.choose(X[ntr:], V[ntr:])
         r_model = regret(Y[ntr:], V[ntr:], ch)
         r_ref = regret(Y[ntr:], V[ntr:], np.zeros(len(ch), np.int64))
-        res = {"train_rows": int(ntr), "test_rows": int(len(ch)), "final_loss": float(loss),
+        fixed = best_fixed(Y[:ntr], V[:ntr])
+        r_fix = regret(Y[ntr:], V[ntr:], np.full(len(ch), fixed, np.int64))
+        res = {"fixed_candidate": fixed, "regret_fixed_mean": float(r_fix.mean()),
+               "regret_fixed_median": float(np.median(r_fix)),"train_rows": int(ntr), "test_rows": int(len(ch)), "final_loss": float(loss),
                "regret_model_mean": float(r_model.mean()), "regret_model_median": float(np.median(r_model)),
                "regret_reference_mean": float(r_ref.mean()), "regret_reference_median": float(np.median(r_ref)),
                "speedup_model_vs_reference_geomean": float(np.exp(np.mean(np.log((1 + r_ref) / (1 + r_model)))))}

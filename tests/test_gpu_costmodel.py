@@ -5,7 +5,8 @@ Data: tests/golden/tune_records.npz, autotuner timings of synthetic jobs
 #candidates, fp32, the library's HIP kernels) is trained on the first rows and
 judged on the last 256: its regret (time of its pick over the measured best)
 against the reference split's (build_giant_kernels, candidate 0), the rule a
-job gets without the model."""
+job gets without the model, and against the best single candidate of the
+training rows (a non-learned rule fitted to the same data)."""
 import os
 
 import numpy as np
@@ -31,10 +32,14 @@ def test_cost_model_beats_reference_rule(ctx):
     choice = m.choose(X[ntr:], V[ntr:])
     r_model = tuning.regret(Y[ntr:], V[ntr:], choice)
     r_ref = tuning.regret(Y[ntr:], V[ntr:], np.zeros(n_test, np.int64))
+    fixed = tuning.best_fixed(Y[:ntr], V[:ntr])
+    r_fix = tuning.regret(Y[ntr:], V[ntr:], np.full(n_test, fixed, np.int64))
     gain = float(np.exp(np.mean(np.log((1 + r_ref) / (1 + r_model)))))
-    print("held-out regret: model mean %.3f median %.3f | reference rule mean %.3f median %.3f | geomean gain %.2fx"
-          % (r_model.mean(), np.median(r_model), r_ref.mean(), np.median(r_ref), gain), flush=True)
+    print("held-out regret: model mean %.3f median %.3f | reference rule mean %.3f median %.3f | best fixed (%d) "
+          "mean %.3f | geomean gain over reference %.2fx" % (r_model.mean(), np.median(r_model), r_ref.mean(),
+                                                              np.median(r_ref), fixed, r_fix.mean(), gain), flush=True)
     assert r_model.mean() < r_ref.mean()
+    assert r_model.mean() < r_fix.mean()
     assert gain > 1.1
     # the untrained model falls back to the reference rule
     fresh = tuning.CostModel(ctx, Y.shape[1])

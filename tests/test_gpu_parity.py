@@ -90,6 +90,9 @@ CASES = [
     ("b2_k100_2e32", 2, 100, 50000, 60000, 2**32 - 10**5, 2**32 + 10**5, 8, 0.3),
     ("b7_k2_2e63", 7, 2, 10, 70000, 2**63 - 10**4, 2**63 + 10**4, 9, 1.0),
     ("b2_k5_top", 2, 5, 1000, 40000, 2**64 - 2 * 10**4, 2**64 - 1, 10, 1.0),
+    # p below / above 2^51: binary64 steps, then integer Montgomery steps
+    ("b2_k12_2e51", 2, 12, 60000, 90000, 2**51 - 10**5, 2**51 + 10**5, 11, 0.7),
+    ("b3_k8_2e50", 3, 8, 20000, 50000, 2**50 - 10**5, 2**50 + 10**5, 12, 1.0),
 ]
 
 
