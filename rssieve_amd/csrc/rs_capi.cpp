@@ -18,6 +18,13 @@
 namespace rs {
 hipError_t bsgs_blocks_per_cu(uint32_t threads, uint32_t hsize, int* out);
 size_t bsgs_lds_bytes(uint32_t hsize, uint32_t threads);
+hipError_t bsgs_static_lds(uint32_t threads, size_t* out);
+// dynamic + static LDS of one BSGS workgroup
+static size_t bsgs_block_lds(uint32_t hsize, uint32_t threads) {
+  size_t st = 8192;  // upper bound if the attribute query fails
+  if (bsgs_static_lds(threads, &st) != hipSuccess) st = 8192;
+  return bsgs_lds_bytes(hsize, threads) + st;
+}
 hipError_t launch_set_cursor(SegJob* sj, uint64_t pmin, uint64_t pmax, uint32_t threads, uint64_t range,
                              uint32_t gen_blocks, hipStream_t s);
 hipError_t launch_round(SegJob* jobs, BatchState* bs, uint32_t njobs, uint32_t gen_blocks, uint32_t setup_blocks,
@@ -247,12 +254,12 @@ static int check_params(const rs_ctx* ctx, const rs_params* prm, const char* who
   if (!(prm->threads == 128 || prm->threads == 256 || prm->threads == 512 || prm->threads == 1024))
     return fail(RS_E_ARG, "%s: threads %u not in {128,256,512,1024}", who, prm->threads);
   const uint32_t entries = prm->algo == RS_ALGO_MERGED ? prm->Q / 2 : prm->Q;  // baby steps in one table
-  if (prm->Q < 16 || entries > 16384 || !pow2(prm->hsize) || prm->hsize < 64 || prm->hsize <= entries)
-    return fail(RS_E_ARG, "%s: need at most 16384 table entries and a power-of-two hsize >= 64 above them "
-                "(Q=%u entries=%u hsize=%u)", who, prm->Q, entries, prm->hsize);
-  if (bsgs_lds_bytes(prm->hsize, prm->threads) + 4096 > ctx->lds_per_block)
+  if (prm->Q < 16 || entries > 16384 || !pow2(prm->hsize) || prm->hsize < 64 || prm->hsize < 2 * entries)
+    return fail(RS_E_ARG, "%s: need at most 16384 table entries and a power-of-two hsize >= 64, at least twice "
+                "the entries (Q=%u entries=%u hsize=%u)", who, prm->Q, entries, prm->hsize);
+  if (bsgs_block_lds(prm->hsize, prm->threads) > ctx->lds_per_block)
     return fail(RS_E_ARG, "%s: hash of %u slots needs %zu LDS bytes, device has %u", who, prm->hsize,
-                bsgs_lds_bytes(prm->hsize, prm->threads), ctx->lds_per_block);
+                bsgs_block_lds(prm->hsize, prm->threads), ctx->lds_per_block);
   return RS_OK;
 }
 
@@ -492,7 +499,7 @@ extern "C" int rs_sieve_batch(rs_job** jobs, int njobs, const uint64_t* pmin, co
     cap_max = std::max(cap_max, j->seg_cap);
     R = std::max(R, rounds_for(pmin[i], pmax[i], range));
   }
-  if (bsgs_lds_bytes(hsize, T) + 4096 > ctx->lds_per_block) return fail(RS_E_ARG, "rs_sieve_batch: table too large");
+  if (bsgs_block_lds(hsize, T) > ctx->lds_per_block) return fail(RS_E_ARG, "rs_sieve_batch: table too large");
   int bpc = 0;
   if (bsgs_blocks_per_cu(T, hsize, &bpc) != hipSuccess || bpc < 1) bpc = 1;
   const uint32_t bsgs_blocks = (uint32_t)bpc * ctx->cus;

@@ -569,14 +569,18 @@ __device__ __forceinline__ void claim(const JobDev* __restrict__ job, uint32_t* 
 }
 
 // ---------------------------------------------------------------------------
-// LDS baby-step table: buckets of 8 slots (32 bytes, two ds_read_b128 per
-// lookup, no probe loop in the common case).  A slot holds an 18-bit
-// fingerprint of the Montgomery residue (never 0) above a 14-bit table entry
-// index; 0 marks an empty slot.  The bucket comes from the residue's low bits,
-// the fingerprint from the others; a fingerprint match is confirmed by
-// recomputing the baby value, so the table answers exactly "which entry holds
-// this residue".  A residue whose low 32 bits are 0 is never inserted: the
-// reference marks empty slots with 0 and cannot find it (giant.cl:44-46, 61-63).
+// LDS baby-step table: buckets of 8 words (32 bytes, two ds_read_b128 per
+// lookup, no probe loop in the common case).  Word 0 counts the values sent
+// to the bucket, words 1..7 hold up to seven of them; a bucket that received
+// more passes the rest to the next bucket.  A slot holds an 18-bit
+// fingerprint of the Montgomery residue above a 14-bit table entry index.
+// The bucket comes from the residue's low bits, the fingerprint from the
+// others; a fingerprint match is confirmed by recomputing the baby value, so
+// the table answers exactly "which entry holds this residue".  Inserting
+// takes one LDS atomic (the count) and one store.  At most hsize/2 values go
+// into hsize words, so 7/8 of the slots always leave room.  A residue whose
+// low 32 bits are 0 is never inserted: the reference marks empty slots with 0
+// and cannot find it (giant.cl:44-46, 61-63).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kBucket = 8;
 
@@ -593,9 +597,11 @@ __device__ __forceinline__ void tab_insert(uint32_t* tab, uint32_t bmask, uint64
   uint32_t b = lo & bmask;
   while (true) {
     uint32_t* bk = tab + b * kBucket;
-#pragma unroll
-    for (uint32_t s = 0; s < kBucket; s++)
-      if (atomicCAS(&bk[s], 0u, word) == 0u) return;
+    const uint32_t c = atomicAdd(bk, 1u);
+    if (c < kBucket - 1) {
+      bk[1 + c] = word;
+      return;
+    }
     b = (b + 1) & bmask;
   }
 }
@@ -616,17 +622,16 @@ __device__ __noinline__ int tab_find_at(const uint32_t* tab, uint32_t bmask, uin
   while (true) {
     const uint4* bk = (const uint4*)(tab + b * kBucket);
     const uint4 x = bk[0], y = bk[1];
-    const uint32_t w[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
-    bool full = true;
+    const uint32_t w[7] = {x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+    const uint32_t n = x.x;
 #pragma unroll
-    for (int s = 0; s < 8; s++) {
-      if (w[s] == 0u) full = false;
-      if ((w[s] >> 14) == fp) {
+    for (uint32_t s = 0; s < 7; s++) {
+      if (s < n && (w[s] >> 14) == fp) {
         const uint32_t e = w[s] & 0x3FFFu;
         if (basepow_sm(mbase, nstart + stride * e, base, p, q, one) == g) return (int)e;
       }
     }
-    if (!full) return -1;
+    if (n < kBucket) return -1;
     b = (b + 1) & bmask;
   }
 }
@@ -635,60 +640,133 @@ __device__ __forceinline__ int tab_find(const TabCtx& t, uint64_t g) {
 }
 
 // Blocked Bloom filter in front of the table: one 32-bit word per table slot,
-// two bits per baby value, both in the word chosen by the residue's low bits.
-// With at most hsize/2 values the false-positive rate is below 2^-10, so a
-// giant step costs one ds_read_b32 and a few integer ops; only the rare
-// positives consult the table.  The key is the low 32 bits of the residue (a
-// residue of a product mod p is spread evenly over them): bits 0..13 pick the
-// word (hsize <= 16384), bits 16..20 and 24..28 the two bits (byte-aligned
-// fields, so each bit is one shift with a sub-dword operand select).
-__device__ __forceinline__ uint32_t flt_bits(uint32_t x) { return (1u << ((x >> 16) & 31u)) | (1u << ((x >> 24) & 31u)); }
-__device__ __forceinline__ void flt_insert(uint32_t* flt, uint32_t fmask, uint32_t x) {
-  atomicOr(&flt[x & fmask], flt_bits(x));
+// three bits per baby value, all in the word chosen by the residue's low bits.
+// With at most hsize/2 values the false-positive rate is about 9e-4 (two bits
+// gave 2.8e-3), so a giant step costs one ds_read_b32 and a few integer ops;
+// only the rare positives consult the table.  Residues of products mod p are
+// spread evenly over their low bits: bits 0..13 pick the word (hsize <=
+// 16384), bits 16..20, 24..28 (byte-aligned fields, one shift each with a
+// sub-dword operand select) and 32..36 the three bits.
+__device__ __forceinline__ uint32_t flt_bits(uint64_t v) {
+  const uint32_t x = (uint32_t)v, y = (uint32_t)(v >> 32);
+  return (1u << ((x >> 16) & 31u)) | (1u << ((x >> 24) & 31u)) | (1u << (y & 31u));
 }
-__device__ __forceinline__ bool flt_maybe(const uint32_t* flt, uint32_t fmask, uint32_t x) {
-  const uint32_t m = flt_bits(x);
-  return (flt[x & fmask] & m) == m;
+__device__ __forceinline__ void flt_insert(uint32_t* flt, uint32_t fmask, uint64_t v) {
+  atomicOr(&flt[(uint32_t)v & fmask], flt_bits(v));
 }
-// the filter bits of x that are clear: 0 iff x may be in the table
-__device__ __forceinline__ uint32_t flt_miss(const uint32_t* flt, uint32_t fmask, uint32_t x) {
-  return ~flt[x & fmask] & flt_bits(x);
+__device__ __forceinline__ bool flt_maybe(const uint32_t* flt, uint32_t fmask, uint64_t v) {
+  const uint32_t m = flt_bits(v);
+  return (flt[(uint32_t)v & fmask] & m) == m;
 }
+// the filter bits of v that are clear: 0 iff v may be in the table
+__device__ __forceinline__ uint32_t flt_miss(const uint32_t* flt, uint32_t fmask, uint64_t v) {
+  return ~flt[(uint32_t)v & fmask] & flt_bits(v);
+}
+
+// Trapezoid self-scheduling of the persistent kernel's work: ticket k (one
+// atomic increment of a shared counter) is the item range [start(k),
+// start(k+1)), chunk sizes falling linearly from f = N/(2B) items (B
+// workgroups) to 1 over C = ceil(2N/(f+1)) tickets, so
+//   start(k) = floor(k f - d k (k-1) / 2),  d = (f-1)/(C-1),
+// clipped to N.  About 4B atomics per launch instead of one per item (one
+// device-wide counter serialises at tens of nanoseconds per atomic), and small
+// chunks at the end balance the finish.  Every block computes the same
+// boundaries from k; consecutive boundaries differ by at least 1 - 1e-9, so
+// the chunks tile [0, N) exactly.
+struct Guide {
+  double f, d;
+  uint32_t N, C;
+};
+__device__ __forceinline__ Guide guide_make(uint32_t N, uint32_t nblocks) {
+  Guide g;
+  g.N = N;
+  g.f = fmax(1.0, floor((double)N / (2.0 * (double)nblocks)));
+  g.C = (uint32_t)ceil(2.0 * (double)N / (g.f + 1.0));
+  g.d = g.C > 1 ? (g.f - 1.0) / (double)(g.C - 1) : 0.0;
+  return g;
+}
+__device__ __forceinline__ uint32_t guide_start(const Guide& g, uint32_t k) {
+  if (k >= g.C) return g.N;
+  const double kd = (double)k;
+  const uint32_t s = (uint32_t)floor(kd * g.f - g.d * kd * (kd - 1.0) * 0.5);
+  return s < g.N ? s : g.N;
+}
+
+// Filter positives are queued in LDS during the giant steps and resolved
+// afterwards by all threads, instead of one lane at a time inside the loop;
+// a full queue falls back to resolving on the spot.
+constexpr uint32_t kQCap = 256;
 
 // One persistent kernel serves every job of a batch.  An item is (job, prime,
 // class) for a job with the reference's three tables (giant.cl giantfull /
 // giantparity) and (job, prime) for a job with the merged table of
 // b^(NMIN + 2j), whose targets carry their shift d.  The LDS holds the largest
 // table of the batch; each item uses its own job's table size.
+// Two 512-thread (or one 1024-thread) workgroups per CU need at most 128
+// VGPRs per lane (four waves per SIMD); smaller groups are held to two per CU
+// by their LDS tables and may use more registers.
 template <int T>
-__global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchState* __restrict__ bs) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 : 1))) void k_bsgs(
+    SegJob* __restrict__ jobs, BatchState* __restrict__ bs) {
+  constexpr int kU = 8;  // targets per giant-step group on the binary64 path at T <= 256
   extern __shared__ uint4 lds4[];
   uint32_t* tab = (uint32_t*)lds4;
   __shared__ uint64_t s_h[2 * RS_MAXK];   // targets of the item
   __shared__ uint32_t s_k[2 * RS_MAXK];   // sequence index | shift << 31
-  __shared__ uint32_t s_item, s_job, s_order, s_dups, s_claims;
+  __shared__ uint32_t s_item, s_job, s_order, s_dups, s_claims, s_qn;
+  __shared__ uint4 s_q[kQCap];  // filter positives of the item: residue, target, giant step
   const uint32_t tid = threadIdx.x;
   const uint32_t njobs = bs->njobs;
   const uint32_t nitems = bs->item_base[njobs];
 
-  // Thread 0 takes items from the queue one ahead: the atomic for the next
-  // item is in flight while this one runs.  A block's items only grow, so
-  // its job index only moves forward.
-  uint32_t nxt = 0, jcur = 0, jend = 0;  // jend: first item past job jcur
+  // Thread 0 takes chunks of items by guided self-scheduling (one atomic per
+  // chunk, see Guide) with the next ticket in flight while a chunk runs.  A
+  // block's items only grow, so its job index only moves forward.
+  const Guide gd = guide_make(nitems, gridDim.x);
+  uint32_t tk = 0, ci = 0, ce = 0;       // ticket in flight, current chunk [ci, ce)
+  uint32_t jcur = 0, jend = 0;           // jend: first item past job jcur
+  // Thread 0 sums the counters of its items in LDS and adds them to the
+  // job's once (when the block moves to the next job and at exit): every
+  // block adding per item would serialise on the few counter addresses.
+  __shared__ unsigned long long s_acc[8];  // dups, claims, work, probes, lists[3], order
+  __shared__ Counters* s_acc_cnt;
+  auto acc_flush = [&]() {
+    Counters* c = s_acc_cnt;
+    if (!c) return;
+    unsigned long long* dst[8] = {(unsigned long long*)&c->dups,       (unsigned long long*)&c->claims,
+                                  (unsigned long long*)&c->work_mmul,  (unsigned long long*)&c->work_probe,
+                                  (unsigned long long*)&c->lists[0],   (unsigned long long*)&c->lists[1],
+                                  (unsigned long long*)&c->lists[2],   (unsigned long long*)&c->order_lists};
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      if (s_acc[i]) atomicAdd(dst[i], s_acc[i]);
+      s_acc[i] = 0;
+    }
+  };
   if (tid == 0) {
-    nxt = atomicAdd(&bs->work_next, 1u);
+    s_acc_cnt = nullptr;
+#pragma unroll
+    for (int i = 0; i < 8; i++) s_acc[i] = 0;
+  }
+  if (tid == 0) {
+    tk = atomicAdd(&bs->work_next, 1u);
     jend = bs->item_base[1];
   }
   for (;;) {
     if (tid == 0) {
-      const uint32_t it = nxt;
-      nxt = atomicAdd(&bs->work_next, 1u);
+      while (ci >= ce && ci < nitems) {  // chunk used up: take the ticket, fetch the next
+        ci = guide_start(gd, tk);
+        ce = guide_start(gd, tk + 1);
+        tk = atomicAdd(&bs->work_next, 1u);
+      }
+      const uint32_t it = ci < ce ? ci++ : nitems;
       if (it < nitems)
         while (it >= jend) jend = bs->item_base[++jcur + 1];
       s_item = it;
       s_job = jcur;
       s_dups = 0;
       s_claims = 0;
+      s_qn = 0;
     }
     __syncthreads();
     const uint32_t item = s_item, jj = s_job;
@@ -723,6 +801,22 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
     FactorRec* __restrict__ fac = J.fac;
     Counters* __restrict__ cnt = J.cnt;
     const TabCtx tc{tab, bmask, mbase, p, q, one, base, nstart, stride};
+    // residue g of target i at giant step qi: find its table entry and claim n
+    auto resolve = [&](uint64_t g, uint32_t i, uint32_t qi) {
+      const int e = tab_find(tc, g);
+      if (e >= 0) {
+        const uint32_t kk = s_k[i];
+        claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu, nstart + qi * Ql + stride * (uint32_t)e + (kk >> 31), &s_dups,
+              &s_claims);
+      }
+    };
+    auto defer = [&](uint64_t g, uint32_t i, uint32_t qi) {
+      const uint32_t slot = atomicAdd(&s_qn, 1u);
+      if (slot < kQCap)
+        s_q[slot] = make_uint4((uint32_t)g, (uint32_t)(g >> 32), i, qi);
+      else
+        resolve(g, i, qi);
+    };
 
     {
       const uint4 z = make_uint4(0u, 0u, 0u, 0u);
@@ -751,7 +845,7 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
         const uint64_t vi = fp_to_u64(v);
         if (j && v == v0d) atomicMin(&s_order, j);
         tab_insert(tab, bmask, vi, e);
-        if ((uint32_t)vi) flt_insert(flt, fmask, (uint32_t)vi);
+        if ((uint32_t)vi) flt_insert(flt, fmask, vi);
         v = fp_mulmod(v, bsd, pd, pinv);
       }
     } else {
@@ -760,7 +854,7 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
         const uint32_t j = e * stride;
         if (j && v == v0) atomicMin(&s_order, j);
         tab_insert(tab, bmask, v, e);
-        if ((uint32_t)v) flt_insert(flt, fmask, (uint32_t)v);
+        if ((uint32_t)v) flt_insert(flt, fmask, v);
         v = mmul(v, bstep, p, q);
       }
     }
@@ -795,7 +889,9 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
     } else if (fp) {
       // giant steps in binary64: the targets (Montgomery residues) times the
       // plain residue of G^q give the Montgomery residues the table holds;
-      // fp_mulmod_bits leaves kFpBias + residue, whose low word is the key
+      // fp_mulmod_bits leaves kFpBias + residue (same low 51 bits).  Thread
+      // tid takes q = tid + r*T in F full rounds, four targets at a time so
+      // four products and four filter reads are in flight together.
       const double pc = pd + kFpTwo52;
       double gq = fp_powmod_sm((double)mmul(R->g[cp], 1, p, q), tid, pd, pinv);
       const double gtd = (double)mmul(R->gT[cp], 1, p, q);
@@ -803,40 +899,61 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
       __syncthreads();
       const uint32_t F = Ml / T;
       for (uint32_t r = 0; r < F; r++) {
-        const uint32_t nq = nstart + (tid + r * T) * Ql;
+        const uint32_t qq = tid + r * T;
         uint32_t i = 0;
-        for (; i + 4 <= nk; i += 4) {
-          const uint64_t u0 = fp_mulmod_bits(__longlong_as_double(s_h[i]), gq, pd, pinv, pc);
-          const uint64_t u1 = fp_mulmod_bits(__longlong_as_double(s_h[i + 1]), gq, pd, pinv, pc);
-          const uint64_t u2 = fp_mulmod_bits(__longlong_as_double(s_h[i + 2]), gq, pd, pinv, pc);
-          const uint64_t u3 = fp_mulmod_bits(__longlong_as_double(s_h[i + 3]), gq, pd, pinv, pc);
-          const uint32_t x0 = flt_miss(flt, fmask, (uint32_t)u0), x1 = flt_miss(flt, fmask, (uint32_t)u1);
-          const uint32_t x2 = flt_miss(flt, fmask, (uint32_t)u2), x3 = flt_miss(flt, fmask, (uint32_t)u3);
-          if (min(min(x0, x1), min(x2, x3)) == 0u) {
-            const uint64_t gg[4] = {u0 ^ kFpBias, u1 ^ kFpBias, u2 ^ kFpBias, u3 ^ kFpBias};
-            const uint32_t xx[4] = {x0, x1, x2, x3};
+        if constexpr (T <= 256) {
+          // Eight targets per group, software-pipelined (the next group's
+          // targets are read while this group's filter words are in flight).
+          // At T <= 256 the LDS allows two waves per SIMD, so the extra
+          // registers cost no occupancy.
+          if (nk >= kU) {
+          double t[kU];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-              if (xx[u]) continue;
-              const int e = tab_find(tc, gg[u]);
-              if (e >= 0) {
-                const uint32_t kk = s_k[i + u];
-                claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu, nq + stride * (uint32_t)e + (kk >> 31), &s_dups,
-                      &s_claims);
-              }
+          for (int k = 0; k < kU; k++) t[k] = __longlong_as_double(s_h[k]);
+          for (; i + kU <= nk; i += kU) {
+            uint64_t u[kU];
+            uint32_t w[kU];
+#pragma unroll
+            for (int k = 0; k < kU; k++) u[k] = fp_mulmod_bits(t[k], gq, pd, pinv, pc);
+#pragma unroll
+            for (int k = 0; k < kU; k++) w[k] = flt[(uint32_t)u[k] & fmask];
+            if (i + 2 * kU <= nk) {
+#pragma unroll
+              for (int k = 0; k < kU; k++) t[k] = __longlong_as_double(s_h[i + kU + k]);
+            }
+            uint32_t x[kU], mn = 0xFFFFFFFFu;
+#pragma unroll
+            for (int k = 0; k < kU; k++) {
+              x[k] = ~w[k] & flt_bits(u[k]);
+              mn = min(mn, x[k]);
+            }
+            if (mn == 0u) {
+#pragma unroll
+              for (int k = 0; k < kU; k++)
+                if (!x[k]) defer(u[k] ^ kFpBias, i + k, qq);
+            }
+          }
+          }
+        } else {
+          // four targets per group, within 128 registers (four waves per SIMD)
+          for (; i + 4 <= nk; i += 4) {
+            const uint64_t u0 = fp_mulmod_bits(__longlong_as_double(s_h[i]), gq, pd, pinv, pc);
+            const uint64_t u1 = fp_mulmod_bits(__longlong_as_double(s_h[i + 1]), gq, pd, pinv, pc);
+            const uint64_t u2 = fp_mulmod_bits(__longlong_as_double(s_h[i + 2]), gq, pd, pinv, pc);
+            const uint64_t u3 = fp_mulmod_bits(__longlong_as_double(s_h[i + 3]), gq, pd, pinv, pc);
+            const uint32_t x0 = flt_miss(flt, fmask, u0), x1 = flt_miss(flt, fmask, u1);
+            const uint32_t x2 = flt_miss(flt, fmask, u2), x3 = flt_miss(flt, fmask, u3);
+            if (min(min(x0, x1), min(x2, x3)) == 0u) {
+              if (!x0) defer(u0 ^ kFpBias, i, qq);
+              if (!x1) defer(u1 ^ kFpBias, i + 1, qq);
+              if (!x2) defer(u2 ^ kFpBias, i + 2, qq);
+              if (!x3) defer(u3 ^ kFpBias, i + 3, qq);
             }
           }
         }
         for (; i < nk; i++) {
           const uint64_t u0 = fp_mulmod_bits(__longlong_as_double(s_h[i]), gq, pd, pinv, pc);
-          if (flt_maybe(flt, fmask, (uint32_t)u0)) {
-            const int e = tab_find(tc, u0 ^ kFpBias);
-            if (e >= 0) {
-              const uint32_t kk = s_k[i];
-              claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu, nq + stride * (uint32_t)e + (kk >> 31), &s_dups,
-                    &s_claims);
-            }
-          }
+          if (flt_maybe(flt, fmask, u0)) defer(u0 ^ kFpBias, i, qq);
         }
         gq = fp_mulmod(gq, gtd, pd, pinv);
       }
@@ -851,14 +968,7 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
         uint32_t j = tid % m, i = tid / m;
         for (uint32_t L = tid; L < m * nk; L += T) {
           const uint64_t u0 = fp_mulmod_bits(__longlong_as_double(s_h[i]), __longlong_as_double(s_g[j]), pd, pinv, pc);
-          if (flt_maybe(flt, fmask, (uint32_t)u0)) {
-            const int e = tab_find(tc, u0 ^ kFpBias);
-            if (e >= 0) {
-              const uint32_t kk = s_k[i];
-              claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu,
-                    nstart + (F * T + j) * Ql + stride * (uint32_t)e + (kk >> 31), &s_dups, &s_claims);
-            }
-          }
+          if (flt_maybe(flt, fmask, u0)) defer(u0 ^ kFpBias, i, F * T + j);
           i += da;
           j += db;
           if (j >= m) {
@@ -870,51 +980,34 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
       work = (unsigned long long)nent + (unsigned long long)Ml * (nk + 1);
       probes = (unsigned long long)Ml * nk;
     } else {
-      // giant steps: thread tid handles q = tid, tid + T, ...; the targets
-      // are taken four at a time so four products and four filter reads are
-      // in flight together
+      // giant steps with integer Montgomery products, same schedule
       const uint64_t g = R->g[cp], gT = R->gT[cp];
       uint64_t Gq = powmod_sm(g, tid, p, q, one);
       const uint32_t F = Ml / T;
       for (uint32_t r = 0; r < F; r++) {
-        const uint32_t nq = nstart + (tid + r * T) * Ql;
+        const uint32_t qq = tid + r * T;
         uint32_t i = 0;
         for (; i + 4 <= nk; i += 4) {
           const uint64_t g0 = mmul(s_h[i], Gq, p, q);
           const uint64_t g1 = mmul(s_h[i + 1], Gq, p, q);
           const uint64_t g2 = mmul(s_h[i + 2], Gq, p, q);
           const uint64_t g3 = mmul(s_h[i + 3], Gq, p, q);
-          const uint32_t x0 = flt_miss(flt, fmask, (uint32_t)g0), x1 = flt_miss(flt, fmask, (uint32_t)g1);
-          const uint32_t x2 = flt_miss(flt, fmask, (uint32_t)g2), x3 = flt_miss(flt, fmask, (uint32_t)g3);
+          const uint32_t x0 = flt_miss(flt, fmask, g0), x1 = flt_miss(flt, fmask, g1);
+          const uint32_t x2 = flt_miss(flt, fmask, g2), x3 = flt_miss(flt, fmask, g3);
           if (min(min(x0, x1), min(x2, x3)) == 0u) {
-            const uint64_t gg[4] = {g0, g1, g2, g3};
-            const uint32_t xx[4] = {x0, x1, x2, x3};
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              if (xx[u]) continue;
-              const int e = tab_find(tc, gg[u]);
-              if (e >= 0) {
-                const uint32_t kk = s_k[i + u];
-                claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu, nq + stride * (uint32_t)e + (kk >> 31), &s_dups,
-                      &s_claims);
-              }
-            }
+            if (!x0) defer(g0, i, qq);
+            if (!x1) defer(g1, i + 1, qq);
+            if (!x2) defer(g2, i + 2, qq);
+            if (!x3) defer(g3, i + 3, qq);
           }
         }
         for (; i < nk; i++) {
           const uint64_t g0 = mmul(s_h[i], Gq, p, q);
-          if (flt_maybe(flt, fmask, (uint32_t)g0)) {
-            const int e = tab_find(tc, g0);
-            if (e >= 0) {
-              const uint32_t kk = s_k[i];
-              claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu, nq + stride * (uint32_t)e + (kk >> 31), &s_dups,
-                    &s_claims);
-            }
-          }
+          if (flt_maybe(flt, fmask, g0)) defer(g0, i, qq);
         }
         Gq = mmul(Gq, gT, p, q);
       }
-      const uint32_t m = Ml - F * T;  // the last giant steps, as in the binary64 branch
+      const uint32_t m = Ml - F * T;
       if (m) {
         if (tid < m) s_g[tid] = Gq;
         __syncthreads();
@@ -922,14 +1015,7 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
         uint32_t j = tid % m, i = tid / m;
         for (uint32_t L = tid; L < m * nk; L += T) {
           const uint64_t g0 = mmul(s_h[i], s_g[j], p, q);
-          if (flt_maybe(flt, fmask, (uint32_t)g0)) {
-            const int e = tab_find(tc, g0);
-            if (e >= 0) {
-              const uint32_t kk = s_k[i];
-              claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu,
-                    nstart + (F * T + j) * Ql + stride * (uint32_t)e + (kk >> 31), &s_dups, &s_claims);
-            }
-          }
+          if (flt_maybe(flt, fmask, g0)) defer(g0, i, F * T + j);
           i += da;
           j += db;
           if (j >= m) {
@@ -942,16 +1028,29 @@ __global__ __launch_bounds__(T) void k_bsgs(SegJob* __restrict__ jobs, BatchStat
       probes = (unsigned long long)Ml * nk;
     }
     __syncthreads();
+    {  // the queued filter positives, one per thread
+      const uint32_t nq = s_qn < kQCap ? s_qn : kQCap;
+      for (uint32_t sidx = tid; sidx < nq; sidx += T) {
+        const uint4 e = s_q[sidx];
+        resolve(((uint64_t)e.y << 32) | e.x, e.z, e.w);
+      }
+    }
+    __syncthreads();
     if (tid == 0) {
-      if (s_dups) atomicAdd((unsigned long long*)&cnt->dups, (unsigned long long)s_dups);
-      if (s_claims) atomicAdd((unsigned long long*)&cnt->claims, (unsigned long long)s_claims);
-      atomicAdd((unsigned long long*)&cnt->work_mmul, work);
-      atomicAdd((unsigned long long*)&cnt->work_probe, probes);
-      atomicAdd((unsigned long long*)&cnt->lists[c], 1ull);
-      if (order != 0xFFFFFFFFu) atomicAdd((unsigned long long*)&cnt->order_lists, 1ull);
+      if (cnt != s_acc_cnt) {
+        acc_flush();
+        s_acc_cnt = cnt;
+      }
+      s_acc[0] += s_dups;
+      s_acc[1] += s_claims;
+      s_acc[2] += work;
+      s_acc[3] += probes;
+      s_acc[4 + c] += 1;
+      if (order != 0xFFFFFFFFu) s_acc[7] += 1;
     }
     __syncthreads();
   }
+  if (tid == 0) acc_flush();
 }
 
 // Work items of every job in prefix order; one thread (a handful of jobs).
@@ -971,6 +1070,21 @@ __global__ void k_round_items(SegJob* __restrict__ jobs, BatchState* __restrict_
 // ---------------------------------------------------------------------------
 // table (hsize u32) + filter (hsize u32) + one u64 power per thread
 size_t bsgs_lds_bytes(uint32_t hsize, uint32_t threads) { return (size_t)hsize * 8 + (size_t)threads * 8; }
+
+// static LDS of the BSGS kernel (target lists, positive queue, counters)
+hipError_t bsgs_static_lds(uint32_t threads, size_t* out) {
+  hipFuncAttributes a;
+  hipError_t e;
+  switch (threads) {
+    case 128: e = hipFuncGetAttributes(&a, (const void*)k_bsgs<128>); break;
+    case 256: e = hipFuncGetAttributes(&a, (const void*)k_bsgs<256>); break;
+    case 512: e = hipFuncGetAttributes(&a, (const void*)k_bsgs<512>); break;
+    case 1024: e = hipFuncGetAttributes(&a, (const void*)k_bsgs<1024>); break;
+    default: return hipErrorInvalidValue;
+  }
+  if (e == hipSuccess) *out = a.sharedSizeBytes;
+  return e;
+}
 
 hipError_t bsgs_blocks_per_cu(uint32_t threads, uint32_t hsize, int* out) {
   const size_t lds = bsgs_lds_bytes(hsize, threads);
