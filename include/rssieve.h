@@ -73,13 +73,17 @@ typedef struct {
 typedef struct {
   int32_t algo;            /* RS_ALGO_* */
   uint32_t Q;              /* baby steps of the full list; parity lists use QQ = 2Q */
-  uint32_t hsize;          /* LDS hash slots, power of 2, >= table entries + 1 */
+  uint32_t hsize;          /* LDS hash slots, power of 2, at least twice the table entries */
   uint32_t threads;        /* BSGS workgroup size: 128, 256, 512 or 1024 */
   uint32_t seg_primes;     /* target primes per segment (reference: 256 * compute units) */
   uint32_t blocks_per_cu;  /* persistent BSGS workgroups per CU (0 = occupancy maximum) */
   int32_t k31_compat;      /* 1: encode k and c as the reference's signed int (k >= 2^31 wraps, inputfile.cpp:366) */
   int32_t record_factors;  /* 0: claims only clear bitmap bits (pre-sieving) */
   uint64_t factor_cap;     /* factor records held on the device between drains */
+  uint32_t queue_cap;      /* filter positives a BSGS workgroup queues before resolving them (0: the
+                              maximum, 512); a queue that overflows is recovered exactly by redoing its
+                              giant steps, so small values only cost time (they exercise that path) */
+  uint32_t reserved;       /* 0 */
 } rs_params;
 
 /* --- context ---------------------------------------------------------- */

@@ -67,6 +67,8 @@ class Params(C.Structure):
         ("k31_compat", C.c_int32),
         ("record_factors", C.c_int32),
         ("factor_cap", C.c_uint64),
+        ("queue_cap", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
     def as_dict(self):

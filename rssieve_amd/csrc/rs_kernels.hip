@@ -590,11 +590,8 @@ __device__ __forceinline__ uint32_t tab_fp(uint64_t v) {
   return f ? f : 1u;
 }
 
-__device__ __forceinline__ void tab_insert(uint32_t* tab, uint32_t bmask, uint64_t v, uint32_t e) {
-  const uint32_t lo = (uint32_t)v;
-  if (lo == 0) return;
-  const uint32_t word = (tab_fp(v) << 14) | e;
-  uint32_t b = lo & bmask;
+// a slot word into the first bucket from b on with room
+__device__ __forceinline__ void tab_insert_word(uint32_t* tab, uint32_t bmask, uint32_t b, uint32_t word) {
   while (true) {
     uint32_t* bk = tab + b * kBucket;
     const uint32_t c = atomicAdd(bk, 1u);
@@ -604,6 +601,12 @@ __device__ __forceinline__ void tab_insert(uint32_t* tab, uint32_t bmask, uint64
     }
     b = (b + 1) & bmask;
   }
+}
+
+__device__ __forceinline__ void tab_insert(uint32_t* tab, uint32_t bmask, uint64_t v, uint32_t e) {
+  const uint32_t lo = (uint32_t)v;
+  if (lo == 0) return;
+  tab_insert_word(tab, bmask, lo & bmask, (tab_fp(v) << 14) | e);
 }
 
 struct TabCtx {
@@ -692,10 +695,10 @@ __device__ __forceinline__ uint32_t guide_start(const Guide& g, uint32_t k) {
   return s < g.N ? s : g.N;
 }
 
-// Filter positives are queued in LDS during the giant steps and resolved
-// afterwards by all threads, instead of one lane at a time inside the loop;
-// a full queue falls back to resolving on the spot.
-constexpr uint32_t kQCap = 256;
+// Filter positives are queued in LDS during the giant steps and resolved by
+// all threads together at round boundaries (once the queue is half full, and
+// at the end of the item), instead of one lane at a time inside the loop.
+constexpr uint32_t kQCap = 512;
 
 // One persistent kernel serves every job of a batch.  An item is (job, prime,
 // class) for a job with the reference's three tables (giant.cl giantfull /
@@ -711,7 +714,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
   constexpr int kU = 8;  // targets per giant-step group on the binary64 path at T <= 256
   extern __shared__ uint4 lds4[];
   uint32_t* tab = (uint32_t*)lds4;
-  __shared__ uint64_t s_h[2 * RS_MAXK];   // targets of the item
+  __shared__ uint64_t s_h[2 * RS_MAXK + 16];  // targets of the item (+ read-ahead pad)
   __shared__ uint32_t s_k[2 * RS_MAXK];   // sequence index | shift << 31
   __shared__ uint32_t s_item, s_job, s_order, s_dups, s_claims, s_qn;
   __shared__ uint4 s_q[kQCap];  // filter positives of the item: residue, target, giant step
@@ -810,12 +813,59 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
               &s_claims);
       }
     };
+    const uint32_t qcap = job->qcap;
+    // Queue a filter positive.  Past kQCap the entry is dropped and its
+    // rounds are redone (settle); the hot loop itself then holds no table
+    // lookups, calls or global accesses.
     auto defer = [&](uint64_t g, uint32_t i, uint32_t qi) {
       const uint32_t slot = atomicAdd(&s_qn, 1u);
-      if (slot < kQCap)
-        s_q[slot] = make_uint4((uint32_t)g, (uint32_t)(g >> 32), i, qi);
-      else
-        resolve(g, i, qi);
+      if (slot < qcap) s_q[slot] = make_uint4((uint32_t)g, (uint32_t)(g >> 32), i, qi);
+    };
+    // Giant steps run in F = Ml / T full rounds (q = tid + r*T) and a last
+    // partial round of m = Ml - F*T steps spread over all threads.
+    const uint32_t F = Ml / T, m = Ml - F * T;
+    // Rounds between queue checks: a round queues about T * nk * 9e-4
+    // positives (the filter's rate at half load); checking when a quarter of
+    // the queue could be filled at 2e-3 keeps overflows rare without a
+    // barrier per round.
+    const uint32_t every = max(1u, 125u * qcap / (T * nk));
+    // Every probe of rounds [ra, rb) (and of the last partial round if rem),
+    // looked up directly with integer Montgomery products h * G^q: the exact
+    // recovery when the queue overflowed and positives were dropped.
+    auto redo = [&](uint32_t ra, uint32_t rb, bool rem) {
+      const uint64_t G = R->g[cp];
+      for (uint32_t rr = ra; rr < rb; rr++) {
+        const uint32_t qi = tid + rr * T;
+        const uint64_t Gq = powmod_sm(G, qi, p, q, one);
+        for (uint32_t i = 0; i < nk; i++) resolve(mmul(kr[i].hadj, Gq, p, q), i, qi);
+      }
+      if (rem)
+        for (uint32_t L = tid; L < m * nk; L += T) {
+          const uint32_t qi = F * T + L % m, i = L / m;
+          resolve(mmul(kr[i].hadj, powmod_sm(G, qi, p, q, one), p, q), i, qi);
+        }
+    };
+    // After the rounds before r1 (block-uniform): resolve the queue if it is
+    // half full or this is the end; redo rounds [r0, r1) if it overflowed.
+    auto settle = [&](uint32_t& r0, uint32_t r1, bool final) {
+      __syncthreads();
+      const uint32_t nq = s_qn;
+      if (nq > qcap) {
+        __syncthreads();
+        if (tid == 0) s_qn = 0;
+        redo(r0, r1, final && m);
+        __syncthreads();
+        r0 = r1;
+      } else if (final || 2 * nq >= qcap) {
+        for (uint32_t sidx = tid; sidx < nq; sidx += T) {
+          const uint4 e = s_q[sidx];
+          resolve(((uint64_t)e.y << 32) | e.x, e.z, e.w);
+        }
+        __syncthreads();
+        if (tid == 0) s_qn = 0;
+        __syncthreads();
+        r0 = r1;
+      }
     };
 
     {
@@ -840,7 +890,36 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       const double bsd = (double)mmul(bstep, 1, p, q);
       const double v0d = (double)v0;
       double v = (double)vstart;
-      for (uint32_t e = tid; e < nent; e += T) {
+      uint32_t e = tid;
+      // four entries (e, e+T, e+2T, e+3T) at a time: their bucket counts are
+      // claimed with four atomics in flight together
+      for (; e + 3 * T < nent; e += 4 * T) {
+        uint64_t vi[4];
+        uint32_t b[4], c[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          vi[k] = fp_to_u64(v);
+          const uint32_t j = (e + k * T) * stride;
+          if (j && v == v0d) atomicMin(&s_order, j);
+          v = fp_mulmod(v, bsd, pd, pinv);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          b[k] = (uint32_t)vi[k] & bmask;
+          if ((uint32_t)vi[k]) c[k] = atomicAdd(&tab[b[k] * kBucket], 1u);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          if (!(uint32_t)vi[k]) continue;
+          const uint32_t word = (tab_fp(vi[k]) << 14) | (e + k * T);
+          if (c[k] < kBucket - 1)
+            tab[b[k] * kBucket + 1 + c[k]] = word;
+          else
+            tab_insert_word(tab, bmask, (b[k] + 1) & bmask, word);
+          flt_insert(flt, fmask, vi[k]);
+        }
+      }
+      for (; e < nent; e += T) {
         const uint32_t j = e * stride;
         const uint64_t vi = fp_to_u64(v);
         if (j && v == v0d) atomicMin(&s_order, j);
@@ -897,42 +976,72 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       const double gtd = (double)mmul(R->gT[cp], 1, p, q);
       for (uint32_t i = tid; i < nk; i += T) s_h[i] = (uint64_t)__double_as_longlong((double)s_h[i]);
       __syncthreads();
-      const uint32_t F = Ml / T;
+      uint32_t r0 = 0;
       for (uint32_t r = 0; r < F; r++) {
         const uint32_t qq = tid + r * T;
         uint32_t i = 0;
         if constexpr (T <= 256) {
-          // Eight targets per group, software-pipelined (the next group's
-          // targets are read while this group's filter words are in flight).
-          // At T <= 256 the LDS allows two waves per SIMD, so the extra
-          // registers cost no occupancy.
-          if (nk >= kU) {
-          double t[kU];
-#pragma unroll
-          for (int k = 0; k < kU; k++) t[k] = __longlong_as_double(s_h[k]);
-          for (; i + kU <= nk; i += kU) {
-            uint64_t u[kU];
-            uint32_t w[kU];
-#pragma unroll
-            for (int k = 0; k < kU; k++) u[k] = fp_mulmod_bits(t[k], gq, pd, pinv, pc);
-#pragma unroll
-            for (int k = 0; k < kU; k++) w[k] = flt[(uint32_t)u[k] & fmask];
-            if (i + 2 * kU <= nk) {
-#pragma unroll
-              for (int k = 0; k < kU; k++) t[k] = __longlong_as_double(s_h[i + kU + k]);
-            }
+          // Groups of eight targets, two in flight: group g's filter words
+          // are read while group g+1's products are computed, and group
+          // g+2's targets are read meanwhile.  At T <= 256 the LDS allows two
+          // waves per SIMD, so the extra registers cost no occupancy.
+          auto consume = [&](const uint64_t(&uu)[kU], const uint32_t(&ww)[kU], uint32_t i0) {
             uint32_t x[kU], mn = 0xFFFFFFFFu;
 #pragma unroll
             for (int k = 0; k < kU; k++) {
-              x[k] = ~w[k] & flt_bits(u[k]);
+              x[k] = ~ww[k] & flt_bits(uu[k]);
               mn = min(mn, x[k]);
             }
             if (mn == 0u) {
 #pragma unroll
               for (int k = 0; k < kU; k++)
-                if (!x[k]) defer(u[k] ^ kFpBias, i + k, qq);
+                if (!x[k]) defer(uu[k] ^ kFpBias, i0 + k, qq);
             }
-          }
+          };
+          // Ping-pong between two register sets (no copies between
+          // iterations, which would wait for the loads); target reads run
+          // one group ahead without a bounds test (s_h is padded).
+          const uint32_t ng = nk / kU;
+          if (ng) {
+            double t[kU];
+            uint64_t uA[kU], uB[kU];
+            uint32_t wA[kU], wB[kU];
+            auto load_t = [&](uint32_t grp) {
+#pragma unroll
+              for (int k = 0; k < kU; k++) t[k] = __longlong_as_double(s_h[grp * kU + k]);
+            };
+            auto mul_t = [&](uint64_t(&uu)[kU]) {
+#pragma unroll
+              for (int k = 0; k < kU; k++) uu[k] = fp_mulmod_bits(t[k], gq, pd, pinv, pc);
+            };
+            auto read_w = [&](const uint64_t(&uu)[kU], uint32_t(&ww)[kU]) {
+#pragma unroll
+              for (int k = 0; k < kU; k++) ww[k] = flt[(uint32_t)uu[k] & fmask];
+            };
+            load_t(0);
+            mul_t(uA);
+            load_t(1);
+            read_w(uA, wA);
+            uint32_t g = 1;
+            for (; g + 1 < ng; g += 2) {  // groups g (B) and g + 1 (A)
+              mul_t(uB);
+              load_t(g + 1);
+              read_w(uB, wB);
+              consume(uA, wA, (g - 1) * kU);
+              mul_t(uA);
+              load_t(g + 2);
+              read_w(uA, wA);
+              consume(uB, wB, g * kU);
+            }
+            if (g < ng) {
+              mul_t(uB);
+              read_w(uB, wB);
+              consume(uA, wA, (g - 1) * kU);
+              consume(uB, wB, g * kU);
+            } else {
+              consume(uA, wA, (g - 1) * kU);
+            }
+            i = ng * kU;
           }
         } else {
           // four targets per group, within 128 registers (four waves per SIMD)
@@ -956,11 +1065,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
           if (flt_maybe(flt, fmask, u0)) defer(u0 ^ kFpBias, i, qq);
         }
         gq = fp_mulmod(gq, gtd, pd, pinv);
+        if ((r + 1) % every == 0) settle(r0, r + 1, false);
       }
       // The last m = Ml - F*T giant steps: their m*nk probes are spread over
       // every thread, (giant step j, target i) for L = tid, tid + T, ...
       // with j = L mod m and i = L / m; thread j < m holds G^(F*T + j).
-      const uint32_t m = Ml - F * T;
       if (m) {
         if (tid < m) s_g[tid] = (uint64_t)__double_as_longlong(gq);
         __syncthreads();
@@ -977,13 +1086,14 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
           }
         }
       }
+      settle(r0, F, true);
       work = (unsigned long long)nent + (unsigned long long)Ml * (nk + 1);
       probes = (unsigned long long)Ml * nk;
     } else {
       // giant steps with integer Montgomery products, same schedule
       const uint64_t g = R->g[cp], gT = R->gT[cp];
       uint64_t Gq = powmod_sm(g, tid, p, q, one);
-      const uint32_t F = Ml / T;
+      uint32_t r0 = 0;
       for (uint32_t r = 0; r < F; r++) {
         const uint32_t qq = tid + r * T;
         uint32_t i = 0;
@@ -1006,8 +1116,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
           if (flt_maybe(flt, fmask, g0)) defer(g0, i, qq);
         }
         Gq = mmul(Gq, gT, p, q);
+        if ((r + 1) % every == 0) settle(r0, r + 1, false);
       }
-      const uint32_t m = Ml - F * T;
       if (m) {
         if (tid < m) s_g[tid] = Gq;
         __syncthreads();
@@ -1024,16 +1134,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
           }
         }
       }
+      settle(r0, F, true);
       work = (unsigned long long)nent + (unsigned long long)Ml * (nk + 1);
       probes = (unsigned long long)Ml * nk;
-    }
-    __syncthreads();
-    {  // the queued filter positives, one per thread
-      const uint32_t nq = s_qn < kQCap ? s_qn : kQCap;
-      for (uint32_t sidx = tid; sidx < nq; sidx += T) {
-        const uint4 e = s_q[sidx];
-        resolve(((uint64_t)e.y << 32) | e.x, e.z, e.w);
-      }
     }
     __syncthreads();
     if (tid == 0) {

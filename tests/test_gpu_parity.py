@@ -139,6 +139,32 @@ def test_sieve_k_above_2_31(ctx, k31):
     _compare(base, ks, cs, nmin, nmax, wps, bm, gpu, orc, k31=k31)
 
 
+OVERFLOW_CASES = [c for c in CASES if c[0] in ("b2_k10_1e9", "b2_k20_2e40", "b7_k2_2e63", "b2_k12_2e51")]
+
+
+@pytest.mark.parametrize("case", OVERFLOW_CASES, ids=[c[0] for c in OVERFLOW_CASES])
+@pytest.mark.parametrize("qcap", [1, 5])
+def test_positive_queue_overflow_is_recovered(ctx, case, qcap):
+    """A BSGS workgroup queues filter positives and resolves them at round
+    boundaries; with a queue of 1 or 5 entries nearly every round overflows
+    and is redone by direct lookups.  Results, duplicate count included, stay
+    the oracle's (binary64 and integer paths, both table shapes)."""
+    name, base, K, nmin, nmax, pmin, pmax, seed, dens = case
+    print("overflow case", case, qcap, flush=True)
+    ks, cs = _draw(base, K, seed)
+    bm, wps = full_bitmap(K, nmin, nmax, density=dens, seed=seed)
+    prm = R.reference_params(nmin, nmax, K, 65536, ctx.compute_units)
+    prm.queue_cap = qcap
+    gpu = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, prm)
+    orc = oracle_run(base, ks, cs, nmin, nmax, bm, pmin, pmax)
+    _compare(base, ks, cs, nmin, nmax, wps, bm, gpu, orc)
+    mprm = merged_params(ctx, nmin, nmax, K, 2048, 4096, 512)
+    mprm.queue_cap = qcap
+    gcnt, gfac, gbm = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, mprm)
+    assert gcnt["claims"] == orc[0]["claims"]
+    assert np.array_equal(gbm, orc[2])
+
+
 def merged_params(ctx, nmin, nmax, K, entries=4096, hsize=8192, threads=256):
     """Parameters of the merged-table path: one table of 'entries' even
     exponent offsets (span 2*entries)."""
