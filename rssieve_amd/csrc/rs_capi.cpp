@@ -257,7 +257,7 @@ static int check_params(const rs_ctx* ctx, const rs_params* prm, const char* who
   if (prm->Q < 16 || entries > 16384 || !pow2(prm->hsize) || prm->hsize < 64 || prm->hsize < 2 * entries)
     return fail(RS_E_ARG, "%s: need at most 16384 table entries and a power-of-two hsize >= 64, at least twice "
                 "the entries (Q=%u entries=%u hsize=%u)", who, prm->Q, entries, prm->hsize);
-  if (prm->queue_cap > 512) return fail(RS_E_ARG, "%s: queue_cap %u above 512", who, prm->queue_cap);
+  if (prm->queue_cap > 384) return fail(RS_E_ARG, "%s: queue_cap %u above 384", who, prm->queue_cap);
   if (bsgs_block_lds(prm->hsize, prm->threads) > ctx->lds_per_block)
     return fail(RS_E_ARG, "%s: hash of %u slots needs %zu LDS bytes, device has %u", who, prm->hsize,
                 bsgs_block_lds(prm->hsize, prm->threads), ctx->lds_per_block);
@@ -278,7 +278,7 @@ static int apply_params(rs_job* j, const rs_params* prm) {
   h.hsize = prm->hsize;
   h.hmask = prm->hsize - 1;
   h.threads = prm->threads;
-  h.qcap = prm->queue_cap ? prm->queue_cap : 512;
+  h.qcap = prm->queue_cap ? prm->queue_cap : 384;
   int bpc = 0;
   if (bsgs_blocks_per_cu(prm->threads, prm->hsize, &bpc) != hipSuccess || bpc < 1) bpc = 1;
   if (prm->blocks_per_cu && (int)prm->blocks_per_cu < bpc) bpc = (int)prm->blocks_per_cu;

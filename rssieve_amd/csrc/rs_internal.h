@@ -24,7 +24,7 @@ struct JobDev {
   uint32_t threads;           // BSGS workgroup size
   uint32_t record_factors;    // 0: claims clear bitmap bits only
   uint64_t factor_cap;
-  uint32_t qcap;              // filter positives queued per BSGS workgroup (1..512)
+  uint32_t qcap;              // filter positives queued per BSGS workgroup (1..384)
   uint32_t pad;
 };
 

@@ -698,26 +698,31 @@ __device__ __forceinline__ uint32_t guide_start(const Guide& g, uint32_t k) {
 // Filter positives are queued in LDS during the giant steps and resolved by
 // all threads together at round boundaries (once the queue is half full, and
 // at the end of the item), instead of one lane at a time inside the loop.
-constexpr uint32_t kQCap = 512;
+// 384 entries keep the static LDS at 13 KB, so a 64 KB filter still fits two
+// 256-thread workgroups per CU.
+constexpr uint32_t kQCap = 384;
+constexpr uint32_t kHQ = 1024;  // power of two, at least twice kQCap
 
 // One persistent kernel serves every job of a batch.  An item is (job, prime,
 // class) for a job with the reference's three tables (giant.cl giantfull /
 // giantparity) and (job, prime) for a job with the merged table of
 // b^(NMIN + 2j), whose targets carry their shift d.  The LDS holds the largest
 // table of the batch; each item uses its own job's table size.
-// Two 512-thread (or one 1024-thread) workgroups per CU need at most 128
-// VGPRs per lane (four waves per SIMD); smaller groups are held to two per CU
-// by their LDS tables and may use more registers.
+// Register budgets for the occupancy the LDS allows at the production table
+// (8192 filter words, 49 KB per workgroup): three 256-thread workgroups per
+// CU (three waves per SIMD, at most 168 VGPRs), two 512-thread or one
+// 1024-thread workgroup (four waves per SIMD, at most 128).
 template <int T>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 : 1))) void k_bsgs(
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 : 3))) void k_bsgs(
     SegJob* __restrict__ jobs, BatchState* __restrict__ bs) {
   constexpr int kU = 8;  // targets per giant-step group on the binary64 path at T <= 256
   extern __shared__ uint4 lds4[];
-  uint32_t* tab = (uint32_t*)lds4;
+  uint32_t* tab = (uint32_t*)lds4;  // the filter; the bucketed table of the overflow recovery reuses it
   __shared__ uint64_t s_h[2 * RS_MAXK + 16];  // targets of the item (+ read-ahead pad)
   __shared__ uint32_t s_k[2 * RS_MAXK];   // sequence index | shift << 31
   __shared__ uint32_t s_item, s_job, s_order, s_dups, s_claims, s_qn;
   __shared__ uint4 s_q[kQCap];  // filter positives of the item: residue, target, giant step
+  __shared__ uint32_t s_hq[kHQ];  // open-addressing index over s_q by residue (slot + 1, 0 = empty)
   const uint32_t tid = threadIdx.x;
   const uint32_t njobs = bs->njobs;
   const uint32_t nitems = bs->item_base[njobs];
@@ -784,8 +789,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     const uint32_t nk = R->nk[c];
     if (nk == 0) continue;
     const uint32_t hsize = job->hsize, bmask = (hsize / kBucket) - 1, fmask = hsize - 1;
-    uint32_t* flt = tab + hsize;
-    uint64_t* s_g = (uint64_t*)(tab + 2 * hsize);  // T giant-step powers, after table and filter
+    uint32_t* flt = tab;
+    uint64_t* s_g = (uint64_t*)(tab + hsize);  // T giant-step powers, after the filter
     const uint32_t base = job->base;
     const int K = job->kcount;
     const uint32_t nmin = job->nmin, nmax = job->nmax;
@@ -803,20 +808,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     uint32_t* __restrict__ bitmap = J.bitmap;
     FactorRec* __restrict__ fac = J.fac;
     Counters* __restrict__ cnt = J.cnt;
-    const TabCtx tc{tab, bmask, mbase, p, q, one, base, nstart, stride};
-    // residue g of target i at giant step qi: find its table entry and claim n
-    auto resolve = [&](uint64_t g, uint32_t i, uint32_t qi) {
-      const int e = tab_find(tc, g);
-      if (e >= 0) {
-        const uint32_t kk = s_k[i];
-        claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu, nstart + qi * Ql + stride * (uint32_t)e + (kk >> 31), &s_dups,
-              &s_claims);
-      }
-    };
     const uint32_t qcap = job->qcap;
-    // Queue a filter positive.  Past kQCap the entry is dropped and its
-    // rounds are redone (settle); the hot loop itself then holds no table
-    // lookups, calls or global accesses.
+    // Queue a filter positive.  Past qcap the entry is dropped and the
+    // item's remaining rounds are redone exactly (settle); the hot loop
+    // itself holds no table lookups, calls or global accesses.
     auto defer = [&](uint64_t g, uint32_t i, uint32_t qi) {
       const uint32_t slot = atomicAdd(&s_qn, 1u);
       if (slot < qcap) s_q[slot] = make_uint4((uint32_t)g, (uint32_t)(g >> 32), i, qi);
@@ -829,48 +824,123 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     // the queue could be filled at 2e-3 keeps overflows rare without a
     // barrier per round.
     const uint32_t every = max(1u, 125u * qcap / (T * nk));
-    // Every probe of rounds [ra, rb) (and of the last partial round if rem),
-    // looked up directly with integer Montgomery products h * G^q: the exact
-    // recovery when the queue overflowed and positives were dropped.
-    auto redo = [&](uint32_t ra, uint32_t rb, bool rem) {
+
+    // Baby steps: entry e = tid + T*i is b^(nstart + stride*e).  Below 2^51
+    // the steps multiply in binary64 by the plain residue of the stride,
+    // which keeps the Montgomery representation of the integer path.
+    const bool fp = (p & 1) && p < kFpLimit;
+    const double pd = (double)p, pinv = fp ? 1.0 / pd : 0.0;
+    const double bd = fp ? (double)mmul(mbase, 1, p, q) : 0.0;  // plain b
+    const uint64_t vstart = fp ? fp_to_u64(fp_mulmod((double)v0, fp_powmod_sm(bd, stride * tid, pd, pinv), pd, pinv))
+                               : mmul(v0, basepow_sm(mbase, stride * tid, base, p, q, one), p, q);
+    const double bsd = fp ? (double)mmul(bstep, 1, p, q) : 0.0;
+    // every baby value of this thread, in order: fn(e, value)
+    auto baby_walk = [&](uint32_t emax, auto&& fn) {
+      if (fp) {
+        double v = (double)vstart;
+        for (uint32_t e = tid; e < emax; e += T) {
+          fn(e, fp_to_u64(v));
+          v = fp_mulmod(v, bsd, pd, pinv);
+        }
+      } else {
+        uint64_t v = vstart;
+        for (uint32_t e = tid; e < emax; e += T) {
+          fn(e, v);
+          v = mmul(v, bstep, p, q);
+        }
+      }
+    };
+
+    // Exact resolution of the queued values s_q[0, nq): an index of them by
+    // residue in LDS, then every baby value e < emax is looked up in it; a
+    // hit is a solution n = nstart + (giant step) * Ql + stride * e + d, or,
+    // with ord != 0 (the order path), n = nstart + stride * e + d + t * ord.
+    // Baby values whose low 32 bits are 0 never match: the reference's table
+    // cannot hold them (giant.cl:44-46, 61-63).
+    auto match = [&](uint32_t nq, uint32_t emax, uint32_t ord) {
+      for (uint32_t s = tid; s < kHQ; s += T) s_hq[s] = 0u;
+      __syncthreads();
+      for (uint32_t s = tid; s < nq; s += T) {
+        uint32_t h = s_q[s].x & (kHQ - 1);
+        while (atomicCAS(&s_hq[h], 0u, s + 1) != 0u) h = (h + 1) & (kHQ - 1);
+      }
+      __syncthreads();
+      baby_walk(emax, [&](uint32_t e, uint64_t v) {
+        const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+        if (!lo) return;
+        uint32_t h = lo & (kHQ - 1), sl;
+        while ((sl = s_hq[h]) != 0u) {
+          const uint4 qe = s_q[sl - 1];
+          if (qe.x == lo && qe.y == hi) {
+            const uint32_t kk = s_k[qe.z], kidx = kk & 0x7FFFFFFFu, d = kk >> 31;
+            if (ord) {
+              for (uint64_t n = (uint64_t)nstart + stride * e + d; n <= nmax; n += ord)
+                claim(job, bitmap, fac, cnt, p, kidx, (uint32_t)n, &s_dups, &s_claims);
+            } else {
+              claim(job, bitmap, fac, cnt, p, kidx, nstart + qe.w * Ql + stride * e + d, &s_dups, &s_claims);
+            }
+          }
+          h = (h + 1) & (kHQ - 1);
+        }
+      });
+    };
+
+    // The overflow recovery: the bucketed table is built over the filter's
+    // LDS and every probe of rounds [ra, F) and of the last partial round is
+    // looked up directly with integer Montgomery products h * G^q.
+    auto redo = [&](uint32_t ra) {
+      __syncthreads();
+      {
+        const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+        for (uint32_t s = tid; s < hsize / 4; s += T) lds4[s] = z;
+      }
+      __syncthreads();
+      baby_walk(nent, [&](uint32_t e, uint64_t v) { tab_insert(tab, bmask, v, e); });
+      __syncthreads();
+      const TabCtx tc{tab, bmask, mbase, p, q, one, base, nstart, stride};
+      auto resolve = [&](uint64_t g, uint32_t i, uint32_t qi) {
+        const int e = tab_find(tc, g);
+        if (e >= 0) {
+          const uint32_t kk = s_k[i];
+          claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu, nstart + qi * Ql + stride * (uint32_t)e + (kk >> 31),
+                &s_dups, &s_claims);
+        }
+      };
       const uint64_t G = R->g[cp];
-      for (uint32_t rr = ra; rr < rb; rr++) {
+      for (uint32_t rr = ra; rr < F; rr++) {
         const uint32_t qi = tid + rr * T;
         const uint64_t Gq = powmod_sm(G, qi, p, q, one);
         for (uint32_t i = 0; i < nk; i++) resolve(mmul(kr[i].hadj, Gq, p, q), i, qi);
       }
-      if (rem)
-        for (uint32_t L = tid; L < m * nk; L += T) {
-          const uint32_t qi = F * T + L % m, i = L / m;
-          resolve(mmul(kr[i].hadj, powmod_sm(G, qi, p, q, one), p, q), i, qi);
-        }
+      for (uint32_t L = tid; L < m * nk; L += T) {
+        const uint32_t qi = F * T + L % m, i = L / m;
+        resolve(mmul(kr[i].hadj, powmod_sm(G, qi, p, q, one), p, q), i, qi);
+      }
     };
-    // After the rounds before r1 (block-uniform): resolve the queue if it is
-    // half full or this is the end; redo rounds [r0, r1) if it overflowed.
-    auto settle = [&](uint32_t& r0, uint32_t r1, bool final) {
+    // After the rounds before r1 (block-uniform): resolve the queue when it
+    // is half full or at the end.  If it overflowed, the positives of rounds
+    // [r0, r1) are incomplete: redo every round from r0 on directly and
+    // report the item finished (true).
+    auto settle = [&](uint32_t& r0, uint32_t r1, bool final) -> bool {
       __syncthreads();
       const uint32_t nq = s_qn;
       if (nq > qcap) {
-        __syncthreads();
-        if (tid == 0) s_qn = 0;
-        redo(r0, r1, final && m);
-        __syncthreads();
-        r0 = r1;
-      } else if (final || 2 * nq >= qcap) {
-        for (uint32_t sidx = tid; sidx < nq; sidx += T) {
-          const uint4 e = s_q[sidx];
-          resolve(((uint64_t)e.y << 32) | e.x, e.z, e.w);
-        }
+        redo(r0);
+        return true;
+      }
+      if (final || 2 * nq >= qcap) {
+        match(nq, nent, 0);
         __syncthreads();
         if (tid == 0) s_qn = 0;
         __syncthreads();
         r0 = r1;
       }
+      return false;
     };
 
     {
       const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-      for (uint32_t s = tid; s < hsize / 2; s += T) lds4[s] = z;  // table and filter
+      for (uint32_t s = tid; s < hsize / 4; s += T) lds4[s] = z;  // the filter
     }
     for (uint32_t i = tid; i < nk; i += T) {
       s_h[i] = kr[i].hadj;
@@ -878,91 +948,26 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     }
     if (tid == 0) s_order = 0xFFFFFFFFu;
     __syncthreads();
-    // baby steps: entry e = tid + T*i holds b^(nstart + stride*e).  Below
-    // 2^51 the steps multiply in binary64 by the plain residue of the stride,
-    // which keeps the Montgomery representation of the integer path.
-    const bool fp = (p & 1) && p < kFpLimit;
-    const double pd = (double)p, pinv = fp ? 1.0 / pd : 0.0;
-    const double bd = fp ? (double)mmul(mbase, 1, p, q) : 0.0;  // plain b
-    const uint64_t vstart = fp ? fp_to_u64(fp_mulmod((double)v0, fp_powmod_sm(bd, stride * tid, pd, pinv), pd, pinv))
-                               : mmul(v0, basepow_sm(mbase, stride * tid, base, p, q, one), p, q);
-    if (fp) {
-      const double bsd = (double)mmul(bstep, 1, p, q);
-      const double v0d = (double)v0;
-      double v = (double)vstart;
-      uint32_t e = tid;
-      // four entries (e, e+T, e+2T, e+3T) at a time: their bucket counts are
-      // claimed with four atomics in flight together
-      for (; e + 3 * T < nent; e += 4 * T) {
-        uint64_t vi[4];
-        uint32_t b[4], c[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          vi[k] = fp_to_u64(v);
-          const uint32_t j = (e + k * T) * stride;
-          if (j && v == v0d) atomicMin(&s_order, j);
-          v = fp_mulmod(v, bsd, pd, pinv);
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          b[k] = (uint32_t)vi[k] & bmask;
-          if ((uint32_t)vi[k]) c[k] = atomicAdd(&tab[b[k] * kBucket], 1u);
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          if (!(uint32_t)vi[k]) continue;
-          const uint32_t word = (tab_fp(vi[k]) << 14) | (e + k * T);
-          if (c[k] < kBucket - 1)
-            tab[b[k] * kBucket + 1 + c[k]] = word;
-          else
-            tab_insert_word(tab, bmask, (b[k] + 1) & bmask, word);
-          flt_insert(flt, fmask, vi[k]);
-        }
-      }
-      for (; e < nent; e += T) {
+    // baby steps set the filter bits and find ord_p(b) below the span
+    {
+      const uint64_t v0m = v0;
+      baby_walk(nent, [&](uint32_t e, uint64_t v) {
         const uint32_t j = e * stride;
-        const uint64_t vi = fp_to_u64(v);
-        if (j && v == v0d) atomicMin(&s_order, j);
-        tab_insert(tab, bmask, vi, e);
-        if ((uint32_t)vi) flt_insert(flt, fmask, vi);
-        v = fp_mulmod(v, bsd, pd, pinv);
-      }
-    } else {
-      uint64_t v = vstart;
-      for (uint32_t e = tid; e < nent; e += T) {
-        const uint32_t j = e * stride;
-        if (j && v == v0) atomicMin(&s_order, j);
-        tab_insert(tab, bmask, v, e);
+        if (j && v == v0m) atomicMin(&s_order, j);
         if ((uint32_t)v) flt_insert(flt, fmask, v);
-        v = mmul(v, bstep, p, q);
-      }
+      });
     }
     __syncthreads();
     const uint32_t order = s_order;
     unsigned long long work, probes;
     if (order != 0xFFFFFFFFu) {
-      // ord_p(b) (or its even multiple) is below the table span: keep only
-      // j < order (giant.cl:121-134), then solve without giant steps.
+      // ord_p(b) (or its even multiple) is below the table span: the baby
+      // values b^(nstart + stride*e) with stride*e < order are distinct and
+      // hold every reachable residue (giant.cl:121-134); the targets are
+      // matched against them directly.
+      for (uint32_t i = tid; i < nk; i += T) s_q[i] = make_uint4((uint32_t)s_h[i], (uint32_t)(s_h[i] >> 32), i, 0u);
       __syncthreads();
-      {
-        const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-        for (uint32_t s = tid; s < hsize / 4; s += T) lds4[s] = z;
-      }
-      __syncthreads();
-      uint64_t v = vstart;
-      for (uint32_t e = tid; e < nent; e += T) {
-        if (e * stride < order) tab_insert(tab, bmask, v, e);
-        v = mmul(v, bstep, p, q);
-      }
-      __syncthreads();
-      for (uint32_t i = tid; i < nk; i += T) {
-        const int e = tab_find(tc, s_h[i]);
-        if (e >= 0) {
-          const uint32_t kidx = s_k[i] & 0x7FFFFFFFu, d = s_k[i] >> 31;
-          for (uint64_t n = (uint64_t)nstart + stride * (uint32_t)e + d; n <= nmax; n += order)
-            claim(job, bitmap, fac, cnt, p, kidx, (uint32_t)n, &s_dups, &s_claims);
-        }
-      }
+      match(nk, (order + stride - 1) / stride, order);
       work = order / stride + nk;
       probes = nk;
     } else if (fp) {
@@ -977,6 +982,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       for (uint32_t i = tid; i < nk; i += T) s_h[i] = (uint64_t)__double_as_longlong((double)s_h[i]);
       __syncthreads();
       uint32_t r0 = 0;
+      bool done = false;  // the overflow recovery finished the item
       for (uint32_t r = 0; r < F; r++) {
         const uint32_t qq = tid + r * T;
         uint32_t i = 0;
@@ -1065,12 +1071,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
           if (flt_maybe(flt, fmask, u0)) defer(u0 ^ kFpBias, i, qq);
         }
         gq = fp_mulmod(gq, gtd, pd, pinv);
-        if ((r + 1) % every == 0) settle(r0, r + 1, false);
+        if ((r + 1) % every == 0 && settle(r0, r + 1, false)) {
+          done = true;
+          break;
+        }
       }
       // The last m = Ml - F*T giant steps: their m*nk probes are spread over
       // every thread, (giant step j, target i) for L = tid, tid + T, ...
       // with j = L mod m and i = L / m; thread j < m holds G^(F*T + j).
-      if (m) {
+      if (!done && m) {
         if (tid < m) s_g[tid] = (uint64_t)__double_as_longlong(gq);
         __syncthreads();
         const uint32_t da = T / m, db = T % m;
@@ -1086,7 +1095,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
           }
         }
       }
-      settle(r0, F, true);
+      if (!done) settle(r0, F, true);
       work = (unsigned long long)nent + (unsigned long long)Ml * (nk + 1);
       probes = (unsigned long long)Ml * nk;
     } else {
@@ -1094,6 +1103,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       const uint64_t g = R->g[cp], gT = R->gT[cp];
       uint64_t Gq = powmod_sm(g, tid, p, q, one);
       uint32_t r0 = 0;
+      bool done = false;  // the overflow recovery finished the item
       for (uint32_t r = 0; r < F; r++) {
         const uint32_t qq = tid + r * T;
         uint32_t i = 0;
@@ -1116,9 +1126,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
           if (flt_maybe(flt, fmask, g0)) defer(g0, i, qq);
         }
         Gq = mmul(Gq, gT, p, q);
-        if ((r + 1) % every == 0) settle(r0, r + 1, false);
+        if ((r + 1) % every == 0 && settle(r0, r + 1, false)) {
+          done = true;
+          break;
+        }
       }
-      if (m) {
+      if (!done && m) {
         if (tid < m) s_g[tid] = Gq;
         __syncthreads();
         const uint32_t da = T / m, db = T % m;
@@ -1134,7 +1147,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
           }
         }
       }
-      settle(r0, F, true);
+      if (!done) settle(r0, F, true);
       work = (unsigned long long)nent + (unsigned long long)Ml * (nk + 1);
       probes = (unsigned long long)Ml * nk;
     }
@@ -1171,8 +1184,8 @@ __global__ void k_round_items(SegJob* __restrict__ jobs, BatchState* __restrict_
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-// table (hsize u32) + filter (hsize u32) + one u64 power per thread
-size_t bsgs_lds_bytes(uint32_t hsize, uint32_t threads) { return (size_t)hsize * 8 + (size_t)threads * 8; }
+// filter (hsize u32) + one u64 power per thread
+size_t bsgs_lds_bytes(uint32_t hsize, uint32_t threads) { return (size_t)hsize * 4 + (size_t)threads * 8; }
 
 // static LDS of the BSGS kernel (target lists, positive queue, counters)
 hipError_t bsgs_static_lds(uint32_t threads, size_t* out) {

@@ -16,6 +16,7 @@ learned cost model built on them.
 Command line (on a GPU):
   python -m rssieve_amd.tuning collect --n 1024 --out records.npz
   python -m rssieve_amd.tuning evaluate --records records.npz
+  python -m rssieve_amd.tuning train --records records.npz   (writes model/cost_model.npz)
 """
 import argparse
 import json
@@ -207,6 +208,10 @@ def main(argv=None):
     e.add_argument("--records", required=True)
     e.add_argument("--test", type=int, default=256)
     e.add_argument("--out", default=None)
+    t = sub.add_parser("train")
+    t.add_argument("--records", required=True)
+    t.add_argument("--out", default=MODEL_PATH)
+    t.add_argument("--seed", type=int, default=1)
     a = ap.parse_args(argv)
     ctx = R.Context(0)
     if a.cmd == "collect":
@@ -215,6 +220,14 @@ def main(argv=None):
         np.savez_compressed(a.out, X=X, Y=Y, V=V, descs=json.dumps(descs),
                             cands=json.dumps([p.as_dict() for p in cands]))
         print("wrote", a.out, X.shape, Y.shape)
+    elif a.cmd == "train":
+        z = np.load(a.records, allow_pickle=False)
+        X, Y, V = z["X"], z["Y"], z["V"]
+        m = CostModel(ctx, Y.shape[1], seed=a.seed)
+        loss = m.fit(X, Y, V, seed=a.seed)
+        save_model(m, a.out, meta={"records": os.path.basename(a.records), "rows": int(len(X)),
+                                   "candidates": int(Y.shape[1]), "final_loss": float(loss)})
+        print(json.dumps({"saved": a.out, "rows": int(len(X)), "final_loss": float(loss)}))
     else:
         z = np.load(a.records, allow_pickle=False)
         X, Y, V = z["X"], z["Y"], z["V"]
