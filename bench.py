@@ -96,6 +96,36 @@ def cpu_baseline(ks, cs, nmin, nmax, bits, target_seconds=15.0):
             "sample": "oracle on p in [2^50+2^38-2^30, +%d): %d primes, %.1f s" % (total_w, primes, spent)}
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes (hipMemcpy), whose FETCH_SIZE and
+    WRITE_SIZE counters scale the BSGS kernel's to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bsgs_cfg2_traffic.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of the BSGS kernel from the committed PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch %s:" % kernel):
+                return float(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def roofline(work_mmul, bsgs_s, launches, kernel):
     """The BSGS kernel against the FP64 roofline: algorithmic modular products
     (the kernel's own count: baby steps, giant-step products, giant-step
@@ -105,7 +135,8 @@ def roofline(work_mmul, bsgs_s, launches, kernel):
     per_launch = work_mmul / max(launches, 1)
     achieved = per_launch * FLOPS_PER_MODMUL / max(avg, 1e-12) / 1e12
     return {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / (FP64_PEAK / 1e12), "traffic": None, "kernel": kernel, "launches": launches,
+            "frac": achieved / (FP64_PEAK / 1e12), "traffic": measured_traffic(kernel), "kernel": kernel,
+            "launches": launches,
             "avg_launch_ms": avg * 1e3, "modmul_per_launch": per_launch,
             "modmul_per_s": per_launch / max(avg, 1e-12)}
 
@@ -218,6 +249,9 @@ def main():
     ap.add_argument("--batch", type=int, default=2, help="views of the job sieved per batched round")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--plain-only", action="store_true")
+    ap.add_argument("--tuned-only", action="store_true", help="skip the plain leg (for profiling the tuned kernel)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first copy this many MiB device to device (a kernel of known traffic for PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the sieve shards by p-range, "
@@ -232,15 +266,19 @@ def main():
     setup_s = time.perf_counter() - t_setup
 
     ctx = R.Context(0)
-    plain = run_plain(ctx, data, args.steps, args.warmup)
-    plain_value = plain["primes"] / plain["seconds"]
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib << 20)
+    plain = None if args.tuned_only else run_plain(ctx, data, args.steps, args.warmup)
     res = plain
     tuned = None
     if not args.plain_only:
         tuned = run_tuned(ctx, data, args.steps, args.warmup, args.batch)
-        if not np.array_equal(tuned["bitmap"], plain["bitmap"]):
+        if plain is not None and not np.array_equal(tuned["bitmap"], plain["bitmap"]):
             sys.exit("bench.py: tuned and plain paths eliminated different terms")
         res = tuned
+    if plain is None:
+        plain = tuned
+    plain_value = plain["primes"] / plain["seconds"]
     ctx.close()
     value = res["primes"] / res["seconds"]
     line = {
@@ -264,7 +302,7 @@ def main():
         "plain_config": plain["cfg"],
         "plain_roofline": dict(plain["roof"], bsgs_busy_frac=plain["busy"]),
         "tuned_over_plain": value / plain_value,
-        "same_terms": tuned is not None,
+        "same_terms": tuned is not None and not args.tuned_only,
         "primes_per_step": res["primes"] / args.steps,
         "factors": int(res["claims"]),
         "setup_s": setup_s,
