@@ -186,7 +186,8 @@ def choose_config(ctx, job, data, log):
     if model is not None and len(cands) == model.n_out:
         nterms = int(np.bitwise_count(bits).sum())
         x = tuning.features(2, K, nterms, nmax - nmin + 1, np.log2(float(P0)))[None, :]
-        idx = int(model.choose(x)[0])
+        valid = np.array([[c.threads != 0 for c in cands]])
+        idx = int(model.choose(x, valid)[0])
         how = "cost model"
         log["model_pick"] = idx
     else:

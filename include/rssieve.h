@@ -73,7 +73,7 @@ typedef struct {
 typedef struct {
   int32_t algo;            /* RS_ALGO_* */
   uint32_t Q;              /* baby steps of the full list; parity lists use QQ = 2Q */
-  uint32_t hsize;          /* LDS hash slots, power of 2, at least twice the table entries */
+  uint32_t hsize;          /* LDS filter words, power of 2, >= 64 and >= the baby-step entries */
   uint32_t threads;        /* BSGS workgroup size: 128, 256, 512 or 1024 */
   uint32_t seg_primes;     /* target primes per segment (reference: 256 * compute units) */
   uint32_t blocks_per_cu;  /* persistent BSGS workgroups per CU (0 = occupancy maximum) */

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -83,6 +84,9 @@ struct rs_ctx {
   // batch staging: SegJob array and work queue shared by rs_sieve_batch calls
   SegJob* d_bjobs = nullptr;
   BatchState* d_bstate = nullptr;
+  // global scratch of the BSGS overflow recovery (one table per workgroup)
+  uint32_t* d_scratch = nullptr;
+  size_t scratch_bytes = 0;
   // captured rounds per batch shape
   std::vector<std::pair<GraphKey, GraphExecPair>> graphs;
 };
@@ -164,6 +168,7 @@ extern "C" void rs_close(rs_ctx* ctx) {
 
   (void)hipFree(ctx->d_bjobs);
   (void)hipFree(ctx->d_bstate);
+  (void)hipFree(ctx->d_scratch);
   delete ctx;
 }
 
@@ -254,9 +259,9 @@ static int check_params(const rs_ctx* ctx, const rs_params* prm, const char* who
   if (!(prm->threads == 128 || prm->threads == 256 || prm->threads == 512 || prm->threads == 1024))
     return fail(RS_E_ARG, "%s: threads %u not in {128,256,512,1024}", who, prm->threads);
   const uint32_t entries = prm->algo == RS_ALGO_MERGED ? prm->Q / 2 : prm->Q;  // baby steps in one table
-  if (prm->Q < 16 || entries > 16384 || !pow2(prm->hsize) || prm->hsize < 64 || prm->hsize < 2 * entries)
-    return fail(RS_E_ARG, "%s: need at most 16384 table entries and a power-of-two hsize >= 64, at least twice "
-                "the entries (Q=%u entries=%u hsize=%u)", who, prm->Q, entries, prm->hsize);
+  if (prm->Q < 16 || entries > 16384 || !pow2(prm->hsize) || prm->hsize < 64 || prm->hsize < entries)
+    return fail(RS_E_ARG, "%s: need at most 16384 baby-step entries and a power-of-two filter of at least 64 "
+                "words and at least one word per entry (Q=%u entries=%u hsize=%u)", who, prm->Q, entries, prm->hsize);
   if (prm->queue_cap > 384) return fail(RS_E_ARG, "%s: queue_cap %u above 384", who, prm->queue_cap);
   if (bsgs_block_lds(prm->hsize, prm->threads) > ctx->lds_per_block)
     return fail(RS_E_ARG, "%s: hash of %u slots needs %zu LDS bytes, device has %u", who, prm->hsize,
@@ -450,6 +455,36 @@ static int record_event(rs_job* j, hipEvent_t* out) {
   return RS_OK;
 }
 
+// Baby-step entries of the largest list of a job's BSGS shape.
+static uint32_t job_max_entries(const JobDev& h) {
+  return h.algo == 1 ? h.Q / 2 : std::max(h.Q, h.QQ / 2);
+}
+
+// Size the overflow-recovery scratch for 'blocks' workgroups and lists of up
+// to 'entries' entries, and point the launch's BatchState at it (stream
+// ordered, before the launches that read it).  Growing it waits for the
+// stream, so no enqueued kernel still uses the old buffer.
+static int ensure_scratch(rs_ctx* ctx, BatchState* d_bs, uint32_t blocks, uint32_t entries) {
+  const uint64_t words = recovery_table_words(entries);
+  const size_t need = (size_t)blocks * words * 4;
+  if (need > ctx->scratch_bytes) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(ctx->d_scratch);
+    ctx->d_scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    if (hipMalloc((void**)&ctx->d_scratch, need) != hipSuccess)
+      return fail(RS_E_NOMEM, "BSGS recovery scratch of %zu bytes", need);
+    ctx->scratch_bytes = need;
+  }
+  struct {
+    uint32_t* p;
+    uint64_t w;
+  } hdr{ctx->d_scratch, words};
+  static_assert(sizeof(hdr) == offsetof(BatchState, njobs), "BatchState header");
+  HIPCHK(hipMemcpyAsync(d_bs, &hdr, sizeof(hdr), hipMemcpyHostToDevice, ctx->stream));
+  return RS_OK;
+}
+
 extern "C" int rs_sieve(rs_job* j, uint64_t pmin, uint64_t pmax) {
   if (!j) return fail(RS_E_ARG, "rs_sieve: null job");
   if (pmin < 3) return fail(RS_E_ARG, "rs_sieve: pmin %llu < 3", (unsigned long long)pmin);
@@ -460,6 +495,10 @@ extern "C" int rs_sieve(rs_job* j, uint64_t pmin, uint64_t pmax) {
   uint32_t gen_blocks;
   segment_shape(j, pmin, &range, &gen_blocks);
   HIPCHK(launch_set_cursor(j->d_sj, pmin, pmax, j->h.threads, range, gen_blocks, s));
+  {
+    const int rc = ensure_scratch(j->ctx, j->d_bs, j->bsgs_blocks, job_max_entries(j->h));
+    if (rc) return rc;
+  }
   if (!j->timer_started) {
     HIPCHK(hipEventRecord(j->ev_all[0], s));
     j->timer_started = true;
@@ -508,6 +547,12 @@ extern "C" int rs_sieve_batch(rs_job** jobs, int njobs, const uint64_t* pmin, co
   const uint32_t setup_blocks = (cap_max + 3) / 4;
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
+  {
+    uint32_t emax = 0;
+    for (int i = 0; i < njobs; i++) emax = std::max(emax, job_max_entries(jobs[i]->h));
+    const int rc = ensure_scratch(ctx, ctx->d_bstate, bsgs_blocks, emax);
+    if (rc) return rc;
+  }
   // the pointers of every job (device to device), then its cursor and shape
   for (int i = 0; i < njobs; i++) {
     HIPCHK(hipMemcpyAsync(ctx->d_bjobs + i, jobs[i]->d_sj, sizeof(SegJob), hipMemcpyDeviceToDevice, s));
@@ -785,10 +830,11 @@ extern "C" int rs_job_reconfigure(rs_job* j, const rs_params* prm) {
   return RS_OK;
 }
 
-// The fixed candidate set of the tuner and the cost model: slot 0 is the
-// reference split; then the merged table at 1024..16384 entries, hash load
-// 1/2 or 1/4 (where the LDS holds it), and 256, 512 or 1024 threads per
-// workgroup.
+// The candidate set of the tuner and the cost model: slot 0 is the reference
+// split; then the merged table at 1024..16384 entries, filter words 1, 2 or 4
+// per entry (where the LDS holds them), and 256, 512 or 1024 threads per
+// workgroup; then merged spans sized so the giant steps fill F = 1..4 whole
+// rounds of the workgroup (the Q of these depends on the n-range).
 extern "C" int rs_tune_candidates(rs_ctx* ctx, uint32_t nmin, uint32_t nmax, int32_t kcount, rs_params* out, int cap,
                                   int* n) {
   if (!ctx || !out || !n || cap < 1) return fail(RS_E_ARG, "rs_tune_candidates: bad args");
@@ -798,7 +844,7 @@ extern "C" int rs_tune_candidates(rs_ctx* ctx, uint32_t nmin, uint32_t nmax, int
   int m = 0;
   out[m++] = ref;
   const uint32_t entries[5] = {1024, 2048, 4096, 8192, 16384};
-  const uint32_t load[2] = {2, 4};
+  const uint32_t load[3] = {1, 2, 4};
   const uint32_t threads[3] = {256, 512, 1024};
   for (uint32_t e : entries)
     for (uint32_t l : load)
@@ -809,6 +855,30 @@ extern "C" int rs_tune_candidates(rs_ctx* ctx, uint32_t nmin, uint32_t nmax, int
         p.hsize = e * l;
         p.threads = t;
         if (check_params(ctx, &p, "rs_tune_candidates") != RS_OK) continue;
+        if (m < cap) out[m] = p;
+        m++;
+      }
+  // Spans whose giant-step count M is an exact multiple of the workgroup size
+  // (F full rounds, no partial last round), filter load 1 or 1/2.  These 24
+  // slots are always listed, so a slot means the same shape for every job; a
+  // slot the n-range or the device cannot use has threads = 0 (rs_autotune
+  // then reports it as not run).
+  const uint64_t L = (uint64_t)nmax - nmin + 1;
+  for (uint32_t t : threads)
+    for (uint32_t F = 1; F <= 4; F++)
+      for (uint32_t l = 1; l <= 2; l++) {
+        const uint64_t Q = 2 * ((L + 2ull * t * F - 1) / (2ull * t * F));
+        const uint64_t e = Q / 2;
+        uint32_t hs = 64;
+        while (hs < e && hs < (1u << 30)) hs <<= 1;
+        rs_params p = ref;
+        p.algo = RS_ALGO_MERGED;
+        p.Q = (uint32_t)std::min<uint64_t>(Q, 1u << 30);
+        p.hsize = hs * l;
+        p.threads = t;
+        if (e < 512 || e > 16384 || (L + Q - 1) / Q != (uint64_t)t * F ||
+            check_params(ctx, &p, "rs_tune_candidates") != RS_OK)
+          p.threads = 0;
         if (m < cap) out[m] = p;
         m++;
       }

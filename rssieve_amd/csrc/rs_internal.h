@@ -96,9 +96,22 @@ struct SegJob {
 
 // Work queue of one round over all jobs of a batch.
 struct BatchState {
+  uint32_t* scratch;       // per-workgroup tables of the BSGS overflow recovery (global memory)
+  uint64_t scratch_words;  // 32-bit words per workgroup
   uint32_t njobs;
   uint32_t work_next;
   uint32_t item_base[RS_MAX_BATCH + 1];  // prefix sums of work items per job
 };
+
+// Words of the overflow-recovery table for a baby-step list of n entries:
+// the least power of two >= 2n (and >= 64), so at most half the slots fill.
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline uint32_t recovery_table_words(uint32_t n) {
+  uint32_t w = 64;
+  while (w < 2 * n) w <<= 1;
+  return w;
+}
 
 }  // namespace rs

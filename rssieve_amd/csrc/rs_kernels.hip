@@ -569,18 +569,18 @@ __device__ __forceinline__ void claim(const JobDev* __restrict__ job, uint32_t* 
 }
 
 // ---------------------------------------------------------------------------
-// LDS baby-step table: buckets of 8 words (32 bytes, two ds_read_b128 per
-// lookup, no probe loop in the common case).  Word 0 counts the values sent
-// to the bucket, words 1..7 hold up to seven of them; a bucket that received
-// more passes the rest to the next bucket.  A slot holds an 18-bit
-// fingerprint of the Montgomery residue above a 14-bit table entry index.
-// The bucket comes from the residue's low bits, the fingerprint from the
-// others; a fingerprint match is confirmed by recomputing the baby value, so
-// the table answers exactly "which entry holds this residue".  Inserting
-// takes one LDS atomic (the count) and one store.  At most hsize/2 values go
-// into hsize words, so 7/8 of the slots always leave room.  A residue whose
-// low 32 bits are 0 is never inserted: the reference marks empty slots with 0
-// and cannot find it (giant.cl:44-46, 61-63).
+// Baby-step table of the overflow recovery (global memory, one per
+// workgroup): buckets of 8 words.  Word 0 counts the values sent to the
+// bucket, words 1..7 hold up to seven of them; a bucket that received more
+// passes the rest to the next bucket.  A slot holds an 18-bit fingerprint of
+// the Montgomery residue above a 14-bit entry index.  The bucket comes from
+// the residue's low bits, the fingerprint from the others; a fingerprint
+// match is confirmed by recomputing the baby value, so the table answers
+// exactly "which entry holds this residue".  At most half the slots fill.  A
+// residue whose low 32 bits are 0 is never inserted: the reference marks
+// empty slots with 0 and cannot find it (giant.cl:44-46, 61-63).  Slot words
+// go through device-scope atomic loads and stores so no stale L1 line is read
+// after the workgroup's inserts.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kBucket = 8;
 
@@ -590,47 +590,37 @@ __device__ __forceinline__ uint32_t tab_fp(uint64_t v) {
   return f ? f : 1u;
 }
 
-// a slot word into the first bucket from b on with room
-__device__ __forceinline__ void tab_insert_word(uint32_t* tab, uint32_t bmask, uint32_t b, uint32_t word) {
+__device__ __forceinline__ uint32_t gld(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gtab_insert(uint32_t* tab, uint32_t bmask, uint64_t v, uint32_t e) {
+  const uint32_t lo = (uint32_t)v;
+  if (lo == 0) return;
+  const uint32_t word = (tab_fp(v) << 14) | e;
+  uint32_t b = lo & bmask;
   while (true) {
     uint32_t* bk = tab + b * kBucket;
     const uint32_t c = atomicAdd(bk, 1u);
     if (c < kBucket - 1) {
-      bk[1 + c] = word;
+      __hip_atomic_store(bk + 1 + c, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return;
     }
     b = (b + 1) & bmask;
   }
 }
-
-__device__ __forceinline__ void tab_insert(uint32_t* tab, uint32_t bmask, uint64_t v, uint32_t e) {
-  const uint32_t lo = (uint32_t)v;
-  if (lo == 0) return;
-  tab_insert_word(tab, bmask, lo & bmask, (tab_fp(v) << 14) | e);
-}
-
-struct TabCtx {
-  const uint32_t* tab;
-  uint32_t bmask;
-  uint64_t mbase, p, q, one;
-  uint32_t base, nstart, stride;
-};
-
-// Entry e whose baby value b^(nstart + stride*e) equals g, or -1.  Out of
-// line (rarely taken) with scalar arguments, so nothing goes through scratch.
-__device__ __noinline__ int tab_find_at(const uint32_t* tab, uint32_t bmask, uint64_t mbase, uint64_t p, uint64_t q,
-                                        uint64_t one, uint32_t base, uint32_t nstart, uint32_t stride, uint64_t g) {
+// Entry e whose baby value b^(nstart + stride*e) equals g, or -1 (rare path,
+// out of line).
+__device__ __noinline__ int gtab_find_at(const uint32_t* tab, uint32_t bmask, uint64_t mbase, uint64_t p, uint64_t q,
+                                         uint64_t one, uint32_t base, uint32_t nstart, uint32_t stride, uint64_t g) {
   const uint32_t fp = tab_fp(g);
   uint32_t b = (uint32_t)g & bmask;
   while (true) {
-    const uint4* bk = (const uint4*)(tab + b * kBucket);
-    const uint4 x = bk[0], y = bk[1];
-    const uint32_t w[7] = {x.y, x.z, x.w, y.x, y.y, y.z, y.w};
-    const uint32_t n = x.x;
-#pragma unroll
-    for (uint32_t s = 0; s < 7; s++) {
-      if (s < n && (w[s] >> 14) == fp) {
-        const uint32_t e = w[s] & 0x3FFFu;
+    const uint32_t* bk = tab + b * kBucket;
+    const uint32_t n = gld(bk);
+    for (uint32_t s = 0; s < 7 && s < n; s++) {
+      const uint32_t w = gld(bk + 1 + s);
+      if ((w >> 14) == fp) {
+        const uint32_t e = w & 0x3FFFu;
         if (basepow_sm(mbase, nstart + stride * e, base, p, q, one) == g) return (int)e;
       }
     }
@@ -638,18 +628,16 @@ __device__ __noinline__ int tab_find_at(const uint32_t* tab, uint32_t bmask, uin
     b = (b + 1) & bmask;
   }
 }
-__device__ __forceinline__ int tab_find(const TabCtx& t, uint64_t g) {
-  return tab_find_at(t.tab, t.bmask, t.mbase, t.p, t.q, t.one, t.base, t.nstart, t.stride, g);
-}
 
-// Blocked Bloom filter in front of the table: one 32-bit word per table slot,
-// three bits per baby value, all in the word chosen by the residue's low bits.
-// With at most hsize/2 values the false-positive rate is about 9e-4 (two bits
-// gave 2.8e-3), so a giant step costs one ds_read_b32 and a few integer ops;
-// only the rare positives consult the table.  Residues of products mod p are
-// spread evenly over their low bits: bits 0..13 pick the word (hsize <=
-// 16384), bits 16..20, 24..28 (byte-aligned fields, one shift each with a
-// sub-dword operand select) and 32..36 the three bits.
+// Blocked Bloom filter of the baby steps: hsize 32-bit words in LDS, three
+// bits per baby value, all in the word chosen by the residue's low bits.
+// With hsize/2 values the false-positive rate is about 9e-4 (two bits gave
+// 2.8e-3), with hsize values about 3e-3, so a giant step costs one
+// ds_read_b32 and a few integer ops; the rare positives are queued and
+// resolved exactly.  Residues of products mod p are spread evenly over their
+// low bits: bits 0..14 pick the word (hsize <= 32768), bits 16..20, 24..28
+// (byte-aligned fields, one shift each with a sub-dword operand select) and
+// 32..36 the three bits.
 __device__ __forceinline__ uint32_t flt_bits(uint64_t v) {
   const uint32_t x = (uint32_t)v, y = (uint32_t)(v >> 32);
   return (1u << ((x >> 16) & 31u)) | (1u << ((x >> 24) & 31u)) | (1u << (y & 31u));
@@ -702,6 +690,7 @@ __device__ __forceinline__ uint32_t guide_start(const Guide& g, uint32_t k) {
 // 256-thread workgroups per CU.
 constexpr uint32_t kQCap = 384;
 constexpr uint32_t kHQ = 1024;  // power of two, at least twice kQCap
+constexpr uint32_t kHitCap = 128;
 
 // One persistent kernel serves every job of a batch.  An item is (job, prime,
 // class) for a job with the reference's three tables (giant.cl giantfull /
@@ -723,6 +712,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
   __shared__ uint32_t s_item, s_job, s_order, s_dups, s_claims, s_qn;
   __shared__ uint4 s_q[kQCap];  // filter positives of the item: residue, target, giant step
   __shared__ uint32_t s_hq[kHQ];  // open-addressing index over s_q by residue (slot + 1, 0 = empty)
+  __shared__ uint2 s_hit[kHitCap];  // baby-step hits of a match pass: (entry, queue slot)
+  __shared__ uint32_t s_nhit;
   const uint32_t tid = threadIdx.x;
   const uint32_t njobs = bs->njobs;
   const uint32_t nitems = bs->item_base[njobs];
@@ -788,7 +779,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     const PrimeRec* __restrict__ R = J.recs + pi;
     const uint32_t nk = R->nk[c];
     if (nk == 0) continue;
-    const uint32_t hsize = job->hsize, bmask = (hsize / kBucket) - 1, fmask = hsize - 1;
+    const uint32_t hsize = job->hsize, fmask = hsize - 1;
     uint32_t* flt = tab;
     uint64_t* s_g = (uint64_t*)(tab + hsize);  // T giant-step powers, after the filter
     const uint32_t base = job->base;
@@ -834,14 +825,26 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     const uint64_t vstart = fp ? fp_to_u64(fp_mulmod((double)v0, fp_powmod_sm(bd, stride * tid, pd, pinv), pd, pinv))
                                : mmul(v0, basepow_sm(mbase, stride * tid, base, p, q, one), p, q);
     const double bsd = fp ? (double)mmul(bstep, 1, p, q) : 0.0;
-    // every baby value of this thread, in order: fn(e, value)
+    // Every baby value of this thread, e = tid + T*i < emax: fn(e, value),
+    // in four interleaved chains (i mod 4) stepping by b^(4*stride*T), so
+    // four independent products are in flight instead of one dependent chain.
     auto baby_walk = [&](uint32_t emax, auto&& fn) {
       if (fp) {
-        double v = (double)vstart;
-        for (uint32_t e = tid; e < emax; e += T) {
-          fn(e, fp_to_u64(v));
-          v = fp_mulmod(v, bsd, pd, pinv);
+        double v[4];
+        v[0] = (double)vstart;
+#pragma unroll
+        for (int c = 1; c < 4; c++) v[c] = fp_mulmod(v[c - 1], bsd, pd, pinv);
+        const double b2 = fp_mulmod(bsd, bsd, pd, pinv), b4 = fp_mulmod(b2, b2, pd, pinv);
+        uint32_t e = tid;
+        for (; e + 3 * T < emax; e += 4 * T) {
+#pragma unroll
+          for (int c = 0; c < 4; c++) fn(e + c * T, fp_to_u64(v[c]));
+#pragma unroll
+          for (int c = 0; c < 4; c++) v[c] = fp_mulmod(v[c], b4, pd, pinv);
         }
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+          if (e + c * T < emax) fn(e + c * T, fp_to_u64(v[c]));
       } else {
         uint64_t v = vstart;
         for (uint32_t e = tid; e < emax; e += T) {
@@ -855,10 +858,24 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     // residue in LDS, then every baby value e < emax is looked up in it; a
     // hit is a solution n = nstart + (giant step) * Ql + stride * e + d, or,
     // with ord != 0 (the order path), n = nstart + stride * e + d + t * ord.
-    // Baby values whose low 32 bits are 0 never match: the reference's table
-    // cannot hold them (giant.cl:44-46, 61-63).
+    // Hits are listed in LDS during the walk and claimed afterwards, so the
+    // walk stays small; a full list sends the whole walk again with claims on
+    // the spot (each hit still claimed once).  Baby values whose low 32 bits
+    // are 0 never match: the reference's table cannot hold them
+    // (giant.cl:44-46, 61-63).
+    auto claim_hit = [&](uint32_t e, uint32_t slot, uint32_t ord) {
+      const uint4 qe = s_q[slot];
+      const uint32_t kk = s_k[qe.z], kidx = kk & 0x7FFFFFFFu, d = kk >> 31;
+      if (ord) {
+        for (uint64_t n = (uint64_t)nstart + stride * e + d; n <= nmax; n += ord)
+          claim(job, bitmap, fac, cnt, p, kidx, (uint32_t)n, &s_dups, &s_claims);
+      } else {
+        claim(job, bitmap, fac, cnt, p, kidx, nstart + qe.w * Ql + stride * e + d, &s_dups, &s_claims);
+      }
+    };
     auto match = [&](uint32_t nq, uint32_t emax, uint32_t ord) {
       for (uint32_t s = tid; s < kHQ; s += T) s_hq[s] = 0u;
+      if (tid == 0) s_nhit = 0;
       __syncthreads();
       for (uint32_t s = tid; s < nq; s += T) {
         uint32_t h = s_q[s].x & (kHQ - 1);
@@ -872,34 +889,50 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         while ((sl = s_hq[h]) != 0u) {
           const uint4 qe = s_q[sl - 1];
           if (qe.x == lo && qe.y == hi) {
-            const uint32_t kk = s_k[qe.z], kidx = kk & 0x7FFFFFFFu, d = kk >> 31;
-            if (ord) {
-              for (uint64_t n = (uint64_t)nstart + stride * e + d; n <= nmax; n += ord)
-                claim(job, bitmap, fac, cnt, p, kidx, (uint32_t)n, &s_dups, &s_claims);
-            } else {
-              claim(job, bitmap, fac, cnt, p, kidx, nstart + qe.w * Ql + stride * e + d, &s_dups, &s_claims);
-            }
+            const uint32_t at = atomicAdd(&s_nhit, 1u);
+            if (at < kHitCap) s_hit[at] = make_uint2(e, sl - 1);
           }
           h = (h + 1) & (kHQ - 1);
         }
       });
+      __syncthreads();
+      const uint32_t nh = s_nhit;
+      if (nh <= kHitCap) {
+        for (uint32_t s = tid; s < nh; s += T) claim_hit(s_hit[s].x, s_hit[s].y, ord);
+      } else {
+        uint64_t v = vstart;  // rare: walk again, claiming on the spot
+        for (uint32_t e = tid; e < emax; e += T) {
+          const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+          if (lo)
+            for (uint32_t h = lo & (kHQ - 1), sl; (sl = s_hq[h]) != 0u; h = (h + 1) & (kHQ - 1))
+              if (s_q[sl - 1].x == lo && s_q[sl - 1].y == hi) claim_hit(e, sl - 1, ord);
+          v = mmul(v, bstep, p, q);
+        }
+      }
     };
 
-    // The overflow recovery: the bucketed table is built over the filter's
-    // LDS and every probe of rounds [ra, F) and of the last partial round is
-    // looked up directly with integer Montgomery products h * G^q.
+    // The overflow recovery: the bucketed table of the baby values is built
+    // in this workgroup's global scratch (at most half full) and every probe
+    // of rounds [ra, F) and of the last partial round is looked up directly
+    // with integer Montgomery products h * G^q.
     auto redo = [&](uint32_t ra) {
+      const uint32_t tw = recovery_table_words(nent), gbmask = tw / kBucket - 1;
+      uint32_t* gtab = bs->scratch + (size_t)blockIdx.x * bs->scratch_words;
       __syncthreads();
-      {
-        const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-        for (uint32_t s = tid; s < hsize / 4; s += T) lds4[s] = z;
+      for (uint32_t s = tid; s < tw; s += T) __hip_atomic_store(gtab + s, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __threadfence();
+      __syncthreads();
+      {  // one chain per thread: this path is rare
+        uint64_t v = vstart;
+        for (uint32_t e = tid; e < nent; e += T) {
+          gtab_insert(gtab, gbmask, v, e);
+          v = mmul(v, bstep, p, q);
+        }
       }
+      __threadfence();
       __syncthreads();
-      baby_walk(nent, [&](uint32_t e, uint64_t v) { tab_insert(tab, bmask, v, e); });
-      __syncthreads();
-      const TabCtx tc{tab, bmask, mbase, p, q, one, base, nstart, stride};
       auto resolve = [&](uint64_t g, uint32_t i, uint32_t qi) {
-        const int e = tab_find(tc, g);
+        const int e = gtab_find_at(gtab, gbmask, mbase, p, q, one, base, nstart, stride, g);
         if (e >= 0) {
           const uint32_t kk = s_k[i];
           claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu, nstart + qi * Ql + stride * (uint32_t)e + (kk >> 31),
@@ -1001,13 +1034,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
             if (mn == 0u) {
 #pragma unroll
               for (int k = 0; k < kU; k++)
-                if (!x[k]) defer(uu[k] ^ kFpBias, i0 + k, qq);
+                if (!x[k] && i0 + k < nk) defer(uu[k] ^ kFpBias, i0 + k, qq);
             }
           };
           // Ping-pong between two register sets (no copies between
           // iterations, which would wait for the loads); target reads run
-          // one group ahead without a bounds test (s_h is padded).
-          const uint32_t ng = nk / kU;
+          // one group ahead without a bounds test (s_h is padded).  The last
+          // group is padded too: its probes past nk are computed and ignored,
+          // so every probe goes through the pipeline.
+          const uint32_t ng = (nk + kU - 1) / kU;
           if (ng) {
             double t[kU];
             uint64_t uA[kU], uB[kU];
@@ -1082,16 +1117,41 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       if (!done && m) {
         if (tid < m) s_g[tid] = (uint64_t)__double_as_longlong(gq);
         __syncthreads();
-        const uint32_t da = T / m, db = T % m;
-        uint32_t j = tid % m, i = tid / m;
-        for (uint32_t L = tid; L < m * nk; L += T) {
-          const uint64_t u0 = fp_mulmod_bits(__longlong_as_double(s_h[i]), __longlong_as_double(s_g[j]), pd, pinv, pc);
-          if (flt_maybe(flt, fmask, u0)) defer(u0 ^ kFpBias, i, F * T + j);
-          i += da;
-          j += db;
-          if (j >= m) {
-            j -= m;
-            i++;
+        // four probes L, L+T, L+2T, L+3T per step, their (j, i) advanced by
+        // 4T; indices past the end are clamped into range and the probes
+        // ignored, so the loop body has no divergent loads
+        const uint32_t total = m * nk, da = (4 * T) / m, db = (4 * T) % m;
+        uint32_t jj4[4], ii4[4];
+        jj4[0] = tid % m;
+        ii4[0] = tid / m;
+#pragma unroll
+        for (int c = 1; c < 4; c++) {
+          jj4[c] = jj4[c - 1] + T % m;
+          ii4[c] = ii4[c - 1] + T / m;
+          if (jj4[c] >= m) {
+            jj4[c] -= m;
+            ii4[c]++;
+          }
+        }
+        for (uint32_t L = tid; L < total; L += 4 * T) {
+          uint64_t u[4];
+          uint32_t w[4];
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            const uint32_t ic = min(ii4[c], nk - 1), jc = min(jj4[c], m - 1);
+            u[c] = fp_mulmod_bits(__longlong_as_double(s_h[ic]), __longlong_as_double(s_g[jc]), pd, pinv, pc);
+          }
+#pragma unroll
+          for (int c = 0; c < 4; c++) w[c] = flt[(uint32_t)u[c] & fmask];
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            if (L + c * T < total && (~w[c] & flt_bits(u[c])) == 0u) defer(u[c] ^ kFpBias, ii4[c], F * T + jj4[c]);
+            ii4[c] += da;
+            jj4[c] += db;
+            if (jj4[c] >= m) {
+              jj4[c] -= m;
+              ii4[c]++;
+            }
           }
         }
       }

@@ -93,8 +93,10 @@ def test_autotune_restores_state_and_best_config_is_exact(ctx):
     print("autotune over", len(cands), "candidates", flush=True)
     best, secs, primes = job.autotune(pmin, width, cands)
     assert 0 <= best < len(cands)
-    assert all(s > 0 for s in secs) and len(set(primes)) == 1 and primes[0] > 0
-    assert secs[best] == min(secs)
+    ran = [c.threads != 0 for c in cands]  # slots this n-range cannot use are marked threads = 0
+    assert all((s > 0) == r for s, r in zip(secs, ran))
+    assert len({pr for pr, r in zip(primes, ran) if r}) == 1 and primes[0] > 0
+    assert secs[best] == min(s for s, r in zip(secs, ran) if r)
     # the tuning left no trace
     assert np.array_equal(job.bitmap(), bm)
     c = job.counters()
