@@ -18,6 +18,9 @@ Two paths sieve the same slices of the same terms:
          into 'batch' views sieved as one batch: one persistent BSGS launch
          per round over all of them, rounds replayed from a HIP graph.
 Their final bitmaps must be identical (checked).  'value' is the tuned path.
+A third leg ('jobs16') sieves sixteen uneven synthetic jobs one at a time
+with the reference split and then all in one batch with the cost model's
+picks, and compares the bitmaps.
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -114,19 +117,20 @@ def copy_calibration(nbytes):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bsgs_cfg2_traffic.txt")
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of the BSGS kernel from the committed PMC summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
+def measured_traffic(path, kernel):
+    """HBM bytes per launch of the BSGS kernel on this path from the committed
+    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or
+    None."""
     try:
         for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch %s:" % kernel):
+            if line.startswith("traffic_bytes_per_launch %s %s:" % (path, kernel)):
                 return float(line.split(":")[1])
     except OSError:
         return None
     return None
 
 
-def roofline(work_mmul, bsgs_s, launches, kernel):
+def roofline(work_mmul, bsgs_s, launches, kernel, path):
     """The BSGS kernel against the FP64 roofline: algorithmic modular products
     (the kernel's own count: baby steps, giant-step products, giant-step
     advances) x 10 flops, per average launch duration (HIP events on the
@@ -135,7 +139,7 @@ def roofline(work_mmul, bsgs_s, launches, kernel):
     per_launch = work_mmul / max(launches, 1)
     achieved = per_launch * FLOPS_PER_MODMUL / max(avg, 1e-12) / 1e12
     return {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / (FP64_PEAK / 1e12), "traffic": measured_traffic(kernel), "kernel": kernel,
+            "frac": achieved / (FP64_PEAK / 1e12), "traffic": measured_traffic(path, kernel), "kernel": kernel,
             "launches": launches,
             "avg_launch_ms": avg * 1e3, "modmul_per_launch": per_launch,
             "modmul_per_s": per_launch / max(avg, 1e-12)}
@@ -165,7 +169,7 @@ def run_plain(ctx, data, steps, warmup):
     job.close()
     M, QQ, MM = R.derive(prm, nmin, nmax)
     return {"primes": c1.primecount - c0.primecount, "seconds": t1 - t0, "bitmap": bm,
-            "roof": roofline(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % prm.threads),
+            "roof": roofline(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % prm.threads, "plain"),
             "busy": bsgs_s / (t1 - t0), "claims": c1.claims, "dups": c1.dups,
             "cfg": {"algo": "three lists", "Q": prm.Q, "M": M, "QQ": QQ, "MM": MM, "hsize": prm.hsize,
                     "threads": prm.threads, "seg_primes": prm.seg_primes}}
@@ -195,6 +199,60 @@ def choose_config(ctx, job, data, log):
         idx = best
         log["autotune_seconds"] = [round(x, 5) for x in secs]
     return cands[idx], how, idx
+
+
+def run_jobs16(ctx, seed=16, width=2**26):
+    """Sixteen uneven jobs (bases 2, 3, 5, 6, 10; 1-100 sequences and n-spans
+    10^4..10^7 log-uniform; k odd below 2^31 coprime to the base; bitmaps of
+    1-10 % density; p from 2^48, one slice of 2^26 each): one job at a time
+    with the reference split, against all sixteen in one batch with the cost
+    model's 256-thread pick per job.  Final bitmaps must agree."""
+    from rssieve_amd import tuning
+    descs = tuning.draw_descriptors(16, seed, log2p_range=(48.0, 48.0))
+    try:
+        model = tuning.load_model(ctx)
+    except Exception:
+        model = None
+
+    cache = {}
+
+    def make_jobs(tuned):
+        jobs = []
+        for d in descs:
+            prm = R.reference_params(d["nmin"], d["nmin"] + d["span"] - 1, d["K"], 65536, ctx.compute_units)
+            job, nterms, nmin, nmax = tuning.synth_job(ctx, d, prm, cache)
+            if tuned and model is not None:
+                cands = ctx.tune_candidates(nmin, nmax, d["K"])
+                if len(cands) == model.n_out:
+                    x = tuning.features(d["base"], d["K"], nterms, d["span"], 48.0)[None, :]
+                    valid = np.array([[c.threads == 256 for c in cands]])
+                    job.reconfigure(cands[int(model.choose(x, valid)[0])])
+            jobs.append(job)
+        return jobs
+
+    p0 = 2**48
+    out = {}
+    for mode in ("sequential", "batched"):
+        jobs = make_jobs(mode == "batched")
+        ctx.sync()
+        t0 = time.perf_counter()
+        if mode == "sequential":
+            for j in jobs:
+                j.sieve(p0, p0 + width)
+        else:
+            R.sieve_batch(jobs, [p0] * len(jobs), [p0 + width] * len(jobs))
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        primes = sum(j.counters().primecount for j in jobs)
+        out[mode] = {"primes_per_s": primes / dt, "seconds": dt, "bitmaps": [j.bitmap() for j in jobs]}
+        for j in jobs:
+            j.close()
+    same = all(np.array_equal(a, b) for a, b in zip(out["sequential"]["bitmaps"], out["batched"]["bitmaps"]))
+    return {"workload": "16 uneven jobs (seed %d), p in [2^48, 2^48 + 2^26) each" % seed,
+            "sequential_reference_primes_per_s": out["sequential"]["primes_per_s"],
+            "batched_tuned_primes_per_s": out["batched"]["primes_per_s"],
+            "batched_over_sequential": out["batched"]["primes_per_s"] / out["sequential"]["primes_per_s"],
+            "same_terms": bool(same), "chosen_by": "cost model" if model is not None else "reference split"}
 
 
 def run_tuned(ctx, data, steps, warmup, batch):
@@ -234,7 +292,7 @@ def run_tuned(ctx, data, steps, warmup, batch):
         v.close()
     job.close()
     return {"primes": c1.primecount - c0.primecount, "seconds": t1 - t0, "bitmap": bm,
-            "roof": roofline(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % cfg.threads),
+            "roof": roofline(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % cfg.threads, "tuned"),
             "busy": bsgs_s / (t1 - t0), "claims": c1.claims, "dups": c1.dups,
             "cfg": {"algo": "merged" if cfg.algo == R.ALGO_MERGED else "three lists", "Q": cfg.Q,
                     "hsize": cfg.hsize, "threads": cfg.threads, "seg_primes": cfg.seg_primes, "batch": batch,
@@ -251,6 +309,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--tuned-only", action="store_true", help="skip the plain leg (for profiling the tuned kernel)")
+    ap.add_argument("--no-jobs16", action="store_true", help="skip the 16-job batch leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a kernel of known traffic for PMC scaling)")
     args = ap.parse_args()
@@ -280,6 +339,9 @@ def main():
     if plain is None:
         plain = tuned
     plain_value = plain["primes"] / plain["seconds"]
+    jobs16 = None
+    if not args.no_jobs16 and not args.plain_only and not args.tuned_only:
+        jobs16 = run_jobs16(ctx)
     ctx.close()
     value = res["primes"] / res["seconds"]
     line = {
@@ -310,6 +372,8 @@ def main():
     }
     if tuned:
         line["tuned_log"] = tuned["log"]
+    if jobs16 is not None:
+        line["jobs16"] = jobs16
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(*data[:5])
     print(json.dumps(line))

@@ -58,10 +58,11 @@ def draw_descriptors(n, seed, log2p_range=(32.0, 62.0)):
     return out
 
 
-def synth_job(ctx, d, params):
+def synth_job(ctx, d, params, cache=None):
     """A job of descriptor d: seeded k coprime to the base (odd for even bases,
     even for odd ones), c alternating, a random bitmap whose bits are set
-    with probability 2^-round(-log2 density)."""
+    with probability 2^-round(-log2 density).  A dict passed as cache keeps
+    the generated bitmaps for later calls with the same descriptor."""
     rng = np.random.default_rng(d["seed"])
     base, K = d["base"], d["K"]
     ks = []
@@ -75,6 +76,10 @@ def synth_job(ctx, d, params):
     nmax = nmin + d["span"] - 1
     wps = (nmax - nmin + 1 + 31) // 32
     # every bit set with probability 2^-r ~ density: the AND of r random words
+    key = (d["seed"], K, nmin, nmax)
+    if cache is not None and key in cache:
+        bits, nterms = cache[key]
+        return ctx.job(base, ks, cs, nmin, nmax, bits, wps, params), nterms, nmin, nmax
     nwords = K * wps
     r = max(1, int(round(-math.log2(d["density"]))))
     bits = rng.integers(0, 2**32, size=nwords, dtype=np.uint32)
@@ -84,6 +89,8 @@ def synth_job(ctx, d, params):
     if tail:
         bits.reshape(K, wps)[:, -1] &= np.uint32((1 << tail) - 1)
     nterms = int(np.bitwise_count(bits).sum())
+    if cache is not None:
+        cache[key] = (bits, nterms)
     return ctx.job(base, ks, cs, nmin, nmax, bits, wps, params), nterms, nmin, nmax
 
 
