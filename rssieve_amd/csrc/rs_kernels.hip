@@ -225,6 +225,9 @@ __global__ __launch_bounds__(256) void k_primegen(SegJob* __restrict__ jobs) {
 __device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d) { return __shfl_up(v, d, 64); }
 __device__ __forceinline__ uint64_t shfl_dn64(uint64_t v, int d) { return __shfl_down(v, d, 64); }
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, int l) { return __shfl(v, l, 64); }
+__device__ __forceinline__ double shfl_up64(double v, int d) { return __shfl_up(v, d, 64); }
+__device__ __forceinline__ double shfl_dn64(double v, int d) { return __shfl_down(v, d, 64); }
+__device__ __forceinline__ double shfl64(double v, int l) { return __shfl(v, l, 64); }
 
 // Two powmods with one shared 64-bit exponent, interleaved for ILP; same
 // masked-shift semantics as powmod_lg.
@@ -266,91 +269,112 @@ __device__ __forceinline__ void fp_powmod_lg2(double x0, double x1, uint64_t exp
 // d of the power-residue tests (setup.cl:22-48)
 __constant__ uint32_t g_prd[12] = {3, 4, 5, 7, 8, 9, 11, 13, 17, 19, 23, 29};
 
-__global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
-  const SegJob& J = jobs[blockIdx.y];
-  const JobDev* __restrict__ job = J.job;
-  const uint64_t* __restrict__ primes = J.primes;
-  PrimeRec* __restrict__ recs = J.recs;
-  KRec* __restrict__ krecs = J.krecs;
-  Counters* __restrict__ cnt = J.cnt;
-  const uint32_t cap = J.cap;
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t pi = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const uint32_t ngen = J.nprimes;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    atomicAdd((unsigned long long*)&cnt->primecount, (unsigned long long)ngen);
-    atomicMax((unsigned long long*)&cnt->max_seg_primes, (unsigned long long)ngen);
-    if (ngen > cap) atomicAdd((unsigned long long*)&cnt->seg_overflow, 1ull);
+// The residue arithmetic of the setup, in two representations with one
+// interface.  Every value the setup computes is a canonical residue, so both
+// give the same stored records bit for bit; only the cost differs.
+//  MontArith: Montgomery residues (x*2^64 mod p) in 64-bit integers, any odd p
+//             (the reference's own representation, common.cl:24-176).
+//  FpArith:   plain residues in binary64 for odd p < 2^51 (exact products,
+//             rs_mont.h), converted to Montgomery form only when stored.
+struct MontArith {
+  using V = uint64_t;
+  uint64_t p, q, one_, r2;
+  __device__ V one() const { return one_; }
+  __device__ V neg_one() const { return p - one_; }
+  __device__ V mul(V a, V b) const { return mmul(a, b, p, q); }
+  __device__ V from_int(uint64_t k) const { return mmul(k, r2, p, q); }  // k < 2^64
+  __device__ V base(uint32_t b) const {
+    const uint64_t two = madd(one_, one_, p);
+    if (b == 2) return two;
+    if (b == 3) return madd(one_, two, p);
+    if (b == 5) return madd(madd(two, two, p), one_, p);
+    return from_int(b);
   }
-  const uint32_t np = ngen < cap ? ngen : cap;
-  if (pi >= np) return;
+  __device__ V base_pow(V mb, uint32_t e, uint32_t b) const { return basepow_sm(mb, e, b, p, q, one_); }
+  __device__ V pow_lg(V x, uint64_t e) const { return powmod_lg(x, e, p, q); }
+  __device__ V pow_sm(V x, uint32_t e) const { return powmod_sm(x, e, p, q, one_); }
+  __device__ void pow_lg2(V x0, V x1, uint64_t e, V* r0, V* r1) const { powmod_lg2(x0, x1, e, p, q, r0, r1); }
+  __device__ uint64_t mont(V x) const { return x; }
+};
 
+struct FpArith {
+  using V = double;
+  double p, pinv, R;  // R = 2^64 mod p
+  __device__ V one() const { return 1.0; }
+  __device__ V neg_one() const { return p - 1.0; }
+  __device__ V mul(V a, V b) const { return fp_mulmod(a, b, p, pinv); }
+  // k mod p.  For k < 2^53 the product k*1 meets the bounds of fp_mulmod
+  // (the quotient estimate is off by less than 1 while k/p < 2^51); larger k
+  // (a sign-extended k in the k >= 2^31 compatibility mode) divide exactly.
+  __device__ V from_int(uint64_t k) const {
+    if (k >> 53) return (double)(k % (uint64_t)p);
+    return fp_mulmod((double)k, 1.0, p, pinv);
+  }
+  __device__ V base(uint32_t b) const { return from_int(b); }
+  __device__ V base_pow(V mb, uint32_t e, uint32_t) const { return fp_powmod_sm(mb, e, p, pinv); }
+  __device__ V pow_lg(V x, uint64_t e) const { return fp_powmod_lg(x, e, p, pinv); }
+  __device__ V pow_sm(V x, uint32_t e) const { return fp_powmod_sm(x, e, p, pinv); }
+  __device__ void pow_lg2(V x0, V x1, uint64_t e, V* r0, V* r1) const { fp_powmod_lg2(x0, x1, e, p, pinv, r0, r1); }
+  __device__ uint64_t mont(V x) const { return fp_to_u64(fp_mulmod(x, R, p, pinv)); }
+};
+
+// The setup of prime p = primes[pi] by one wavefront (setup.cl:89-209 and
+// sort.cl:11-104).
+template <class A>
+__device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64_t p, uint64_t q, uint64_t one,
+                                            uint32_t pi, uint32_t lane) {
+  using V = typename A::V;
+  const JobDev* __restrict__ job = J.job;
+  PrimeRec* __restrict__ R = J.recs + pi;
+  Counters* __restrict__ cnt = J.cnt;
   const uint32_t base = job->base;
   const int K = job->kcount;
   const uint32_t T = J.threads;
-  const uint64_t p = primes[pi];
-  const uint64_t q = mont_inv(p);
-  const uint64_t one = r_mod(p);
-  const uint64_t nmo = p - one;
-  const uint64_t two = madd(one, one, p);
-  const uint64_t four = madd(two, two, p);
-  uint64_t r2 = four;
-#pragma unroll
-  for (int i = 0; i < 5; i++) r2 = mmul(r2, r2, p, q);  // 4^(2^5) = 2^64: Montgomery form of R
-  uint64_t mbase;
-  if (base == 2) mbase = two;
-  else if (base == 3) mbase = madd(one, two, p);
-  else if (base == 5) mbase = madd(four, one, p);
-  else mbase = mmul(base, r2, p, q);
+  const V vone = ar.one(), nmo = ar.neg_one();
+  const V mbase = ar.base(base);
 
-  // Montgomery k for k = lane and k = lane + 64; 'one' pads the products.
-  const uint64_t mk0 = (int)lane < K ? mmul(job->ak[lane], r2, p, q) : one;
-  const uint64_t mk1 = (int)lane + 64 < K ? mmul(job->ak[lane + 64], r2, p, q) : one;
+  // k for k = lane and k = lane + 64; 'one' pads the products.
+  const V mk0 = (int)lane < K ? ar.from_int(job->ak[lane]) : vone;
+  const V mk1 = (int)lane + 64 < K ? ar.from_int(job->ak[lane + 64]) : vone;
   // The invertibility test multiplies the k by b^Qref, Qref being the
   // reference's own split for this n-range (setup.cl:137-162): for a composite
   // 2-PRP the test's outcome depends on that exponent, so every BSGS shape
   // uses the same one and skips the same numbers.
-  const uint64_t gQ = basepow_sm(mbase, job->Qref, base, p, q, one);
+  const V gQ = ar.base_pow(mbase, job->Qref, base);
 
   // inclusive prefix (s) and suffix (u) products over the wavefront
-  uint64_t s0 = mk0, s1 = mk1, u0 = mk0, u1 = mk1;
+  V s0 = mk0, s1 = mk1, u0 = mk0, u1 = mk1;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
-    const uint64_t a0 = shfl_up64(s0, d), a1 = shfl_up64(s1, d);
-    const uint64_t b0 = shfl_dn64(u0, d), b1 = shfl_dn64(u1, d);
+    const V a0 = shfl_up64(s0, d), a1 = shfl_up64(s1, d);
+    const V b0 = shfl_dn64(u0, d), b1 = shfl_dn64(u1, d);
     if ((int)lane >= d) {
-      s0 = mmul(s0, a0, p, q);
-      s1 = mmul(s1, a1, p, q);
+      s0 = ar.mul(s0, a0);
+      s1 = ar.mul(s1, a1);
     }
     if ((int)lane + d < 64) {
-      u0 = mmul(u0, b0, p, q);
-      u1 = mmul(u1, b1, p, q);
+      u0 = ar.mul(u0, b0);
+      u1 = ar.mul(u1, b1);
     }
   }
-  const uint64_t A = shfl64(s0, 63), B = shfl64(s1, 63);
-  const uint64_t ep0 = shfl_up64(s0, 1), ep1 = shfl_up64(s1, 1);
-  const uint64_t es0 = shfl_dn64(u0, 1), es1 = shfl_dn64(u1, 1);
+  const V A0 = shfl64(s0, 63), B0 = shfl64(s1, 63);
+  const V ep0 = shfl_up64(s0, 1), ep1 = shfl_up64(s1, 1);
+  const V es0 = shfl_dn64(u0, 1), es1 = shfl_dn64(u1, 1);
   // products of all other k: before and after, for both slots of this lane
-  const uint64_t pre0 = lane ? ep0 : one;
-  const uint64_t pre1 = mmul(A, lane ? ep1 : one, p, q);
-  const uint64_t suf0 = mmul(lane < 63 ? es0 : one, B, p, q);
-  const uint64_t suf1 = lane < 63 ? es1 : one;
-  const uint64_t AB = mmul(A, B, p, q);
-  const uint64_t total = mmul(AB, gQ, p, q);  // prefix[KCOUNT] of setup.cl:147-151
+  const V pre0 = lane ? ep0 : vone;
+  const V pre1 = ar.mul(A0, lane ? ep1 : vone);
+  const V suf0 = ar.mul(lane < 63 ? es0 : vone, B0);
+  const V suf1 = lane < 63 ? es1 : vone;
+  const V AB = ar.mul(A0, B0);
+  const V total = ar.mul(AB, gQ);  // prefix[KCOUNT] of setup.cl:147-151
 
   // Lane-parallel exponentiations, one code path (no serialized branches):
   //  lane 0: Fermat inverse total^(p-2)          (setup.cl:154)
   //  lanes 1..12: base^((p-1)/d) for d | p-1       (setup.cl:22-48)
   //  lane 13: base^((p-1)/2)                       (setup.cl:51-53)
-  // Odd p below 2^51 exponentiate in binary64 on plain residues (x*R^-1,
-  // one Montgomery product away) and return to Montgomery form through R^2.
-  const bool fp = (p & 1) && p < kFpLimit;
-  const double pd = (double)p, pinv = fp ? 1.0 / pd : 0.0;
-  auto to_plain = [&](uint64_t m) { return (double)mmul(m, 1, p, q); };
-  auto to_mont = [&](double x) { return mmul(fp_to_u64(x), r2, p, q); };
-
   const uint64_t pm = p - 1;
-  uint64_t ex = 0, x = mbase;
+  uint64_t ex = 0;
+  V x = mbase;
   bool dvalid = false;
   if (lane == 0) {
     ex = p - 2;
@@ -363,15 +387,14 @@ __global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
   } else if (lane == 13) {
     ex = pm >> 1;
   }
-  uint64_t r = one;
-  if (ex) r = fp ? to_mont(fp_powmod_lg(to_plain(x), ex, pd, pinv)) : powmod_lg(x, ex, p, q);
-  const uint64_t inv_total = shfl64(r, 0);
-  const bool invertible = mmul(total, inv_total, p, q) == one;  // setup.cl:156-162
-  const uint64_t dmask = __ballot(dvalid && r == one) >> 1;      // passing d tests, bit i = g_prd[i]
-  const uint64_t rg = shfl64(r, 13);
-  const int resg = (rg == one) - (rg == nmo);
+  V r = vone;
+  if (ex) r = ar.pow_lg(x, ex);
+  const V inv_total = shfl64(r, 0);
+  const bool invertible = ar.mul(total, inv_total) == vone;  // setup.cl:156-162
+  const uint64_t dmask = __ballot(dvalid && r == vone) >> 1;  // passing d tests, bit i = g_prd[i]
+  const V rg = shfl64(r, 13);
+  const int resg = (rg == vone) - (rg == nmo);
 
-  PrimeRec* R = recs + pi;
   if (!invertible) {
     if (lane == 0) {
       R->nk[0] = R->nk[1] = R->nk[2] = 0;
@@ -379,52 +402,44 @@ __global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
     }
     return;
   }
-  const uint64_t gQref_inv = mmul(inv_total, AB, p, q);  // b^-Qref, setup.cl:164-165
-  const uint64_t inv_k = mmul(inv_total, gQ, p, q);   // (prod of all k)^-1
+  const V gQref_inv = ar.mul(inv_total, AB);  // b^-Qref, setup.cl:164-165
+  const V inv_k = ar.mul(inv_total, gQ);      // (prod of all k)^-1
 
   // k^-1 and the target hadj for both slots (setup.cl:180-188)
-  const uint64_t kinv0 = mmul(inv_k, mmul(pre0, suf0, p, q), p, q);
-  const uint64_t kinv1 = mmul(inv_k, mmul(pre1, suf1, p, q), p, q);
+  const V kinv0 = ar.mul(inv_k, ar.mul(pre0, suf0));
+  const V kinv1 = ar.mul(inv_k, ar.mul(pre1, suf1));
   const bool act0 = (int)lane < K, act1 = (int)lane + 64 < K;
-  const uint64_t h0 = (act0 && job->cpos[lane]) ? mmul(nmo, kinv0, p, q) : kinv0;
-  const uint64_t h1 = (act1 && job->cpos[lane + 64]) ? mmul(nmo, kinv1, p, q) : kinv1;
+  const V h0 = (act0 && job->cpos[lane]) ? ar.mul(nmo, kinv0) : kinv0;
+  const V h1 = (act1 && job->cpos[lane + 64]) ? ar.mul(nmo, kinv1) : kinv1;
 
-  // prefilter (setup.cl:59-87); the order of the tests does not change the result
-  // h^e for both slots, as +1 (one), -1 (p - 1) or 0 (anything else)
-  const double hd0 = fp ? to_plain(h0) : 0.0, hd1 = fp ? to_plain(h1) : 0.0;
-  auto pow_sign2 = [&](uint64_t e, int* s0, int* s1) {
-    if (fp) {
-      double a0, a1;
-      fp_powmod_lg2(hd0, hd1, e, pd, pinv, &a0, &a1);
-      *s0 = (a0 == 1.0) - (a0 == pd - 1.0);
-      *s1 = (a1 == 1.0) - (a1 == pd - 1.0);
-    } else {
-      uint64_t a0, a1;
-      powmod_lg2(h0, h1, e, p, q, &a0, &a1);
-      *s0 = (a0 == one) - (a0 == nmo);
-      *s1 = (a1 == one) - (a1 == nmo);
-    }
+  // prefilter (setup.cl:59-87); the order of the tests does not change the
+  // result.  h^e for both slots, as +1, -1 or 0 (anything else).
+  auto pow_sign2 = [&](uint64_t e, int* t0, int* t1) {
+    V a0, a1;
+    ar.pow_lg2(h0, h1, e, &a0, &a1);
+    *t0 = (a0 == vone) - (a0 == nmo);
+    *t1 = (a1 == vone) - (a1 == nmo);
   };
   int cls0 = act0 ? 1 : 0, cls1 = act1 ? 1 : 0;
   uint64_t dm = dmask;
   while (dm) {
     const int i = __ffsll((long long)dm) - 1;
     dm &= dm - 1;
-    int s0, s1;
-    pow_sign2(shfl64(ex, i + 1), &s0, &s1);
-    if (s0 != 1) cls0 = 0;
-    if (s1 != 1) cls1 = 0;
+    int t0, t1;
+    pow_sign2(shfl64(ex, i + 1), &t0, &t1);
+    if (t0 != 1) cls0 = 0;
+    if (t1 != 1) cls1 = 0;
   }
   {
-    int s0, s1;
-    pow_sign2(pm >> 1, &s0, &s1);
+    int t0, t1;
+    pow_sign2(pm >> 1, &t0, &t1);
     if (cls0) {
-      if (resg == -1) cls0 = (s0 == 1) ? 2 : (s0 == -1) ? 3 : 0;
-      else if (resg == 1 && s0 == -1) cls0 = 0;
+      if (resg == -1) cls0 = (t0 == 1) ? 2 : (t0 == -1) ? 3 : 0;
+      else if (resg == 1 && t0 == -1) cls0 = 0;
     }
     if (cls1) {
-      if (resg == -1) cls1 = (s1 == 1) ? 2 : (s1 == -1) ? 3 : 0;
-      else if (resg == 1 && s1 == -1) cls1 = 0;
+      if (resg == -1) cls1 = (t1 == 1) ? 2 : (t1 == -1) ? 3 : 0;
+      else if (resg == 1 && t1 == -1) cls1 = 0;
     }
   }
 
@@ -440,29 +455,29 @@ __global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
   // Constants of the BSGS kernel, lane-parallel in two rounds:
   //  lanes 0, 3, 4, 5: b^NMIN, b^T, b^(2T), b^(Qref-1)  ->  b^-1 = b^(Qref-1) * b^-Qref
   //  lanes 0, 1, 2:    b^-Q, b^-QT, b^-2QT (giant step of the table span Q)
-  uint64_t y = one, yb = mbase;
+  V y = vone;
   uint32_t ye = 0;
   if (lane == 0) ye = job->nmin;
   else if (lane == 3) ye = T;
   else if (lane == 4) ye = 2 * T;
   else if (lane == 5) ye = job->Qref - 1;
-  if (lane == 0 || (lane >= 3 && lane <= 5))
-    y = fp ? to_mont(fp_powmod_sm(to_plain(yb), ye, pd, pinv)) : powmod_sm(yb, ye, p, q, one);
-  const uint64_t bn = shfl64(y, 0), bT0 = shfl64(y, 3), bT1 = shfl64(y, 4);
-  const uint64_t binv = mmul(shfl64(y, 5), gQref_inv, p, q);
+  if (lane == 0 || (lane >= 3 && lane <= 5)) y = ar.pow_sm(mbase, ye);
+  const V bn = shfl64(y, 0), bT0 = shfl64(y, 3), bT1 = shfl64(y, 4);
+  const V binv = ar.mul(shfl64(y, 5), gQref_inv);
   ye = 0;
   if (lane == 0) ye = job->Q;
   else if (lane == 1) ye = job->Q * T;
   else if (lane == 2) ye = 2 * job->Q * T;
-  y = one;
-  if (lane < 3) y = fp ? to_mont(fp_powmod_sm(to_plain(binv), ye, pd, pinv)) : powmod_sm(binv, ye, p, q, one);
-  const uint64_t gQ_inv = shfl64(y, 0), gT0 = shfl64(y, 1), gT1 = shfl64(y, 2);
-  const uint64_t g1 = mmul(gQ_inv, gQ_inv, p, q);
+  y = vone;
+  if (lane < 3) y = ar.pow_sm(binv, ye);
+  const V gQ_inv = shfl64(y, 0), gT0 = shfl64(y, 1), gT1 = shfl64(y, 2);
+  const V g1 = ar.mul(gQ_inv, gQ_inv);
 
   uint32_t nk[3];
-  KRec* kr = krecs + (size_t)pi * (merged ? 2 * K : K);
+  KRec* kr = J.krecs + (size_t)pi * (merged ? 2 * K : K);
   if (!merged) {
     // compaction by class: full, even, odd (the reference's sort.cl:40-71)
+    const uint64_t hm0 = ar.mont(h0), hm1 = ar.mont(h1);
     uint32_t off = 0;
 #pragma unroll
     for (int c = 1; c <= 3; c++) {
@@ -470,14 +485,14 @@ __global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
       const uint32_t c0 = __popcll(m0), c1 = __popcll(m1);
       if (cls0 == c) {
         KRec e;
-        e.hadj = h0;
+        e.hadj = hm0;
         e.kidx = lane;
         e.cls = c;
         kr[off + lane_rank(m0)] = e;
       }
       if (cls1 == c) {
         KRec e;
-        e.hadj = h1;
+        e.hadj = hm1;
         e.kidx = lane + 64;
         e.cls = c;
         kr[off + c0 + lane_rank(m1)] = e;
@@ -497,14 +512,14 @@ __global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
       const uint32_t c0 = __popcll(m0), c1 = __popcll(m1);
       if (s0) {
         KRec e;
-        e.hadj = d ? mmul(h0, binv, p, q) : h0;
+        e.hadj = ar.mont(d ? ar.mul(h0, binv) : h0);
         e.kidx = lane;
         e.cls = d;
         kr[off + lane_rank(m0)] = e;
       }
       if (s1) {
         KRec e;
-        e.hadj = d ? mmul(h1, binv, p, q) : h1;
+        e.hadj = ar.mont(d ? ar.mul(h1, binv) : h1);
         e.kidx = lane + 64;
         e.cls = d;
         kr[off + c0 + lane_rank(m1)] = e;
@@ -514,25 +529,73 @@ __global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
     nk[0] = off;
     nk[1] = nk[2] = 0;
   }
-  if (lane == 0) {
-    PrimeRec rec;
-    rec.p = p;
-    rec.q = q;
-    rec.one = one;
-    rec.mbase = mbase;
-    rec.bn = bn;
-    rec.g[0] = gQ_inv;
-    rec.g[1] = g1;
-    rec.gT[0] = gT0;
-    rec.gT[1] = gT1;
-    rec.bT[0] = merged ? bT1 : bT0;
-    rec.bT[1] = bT1;
-    rec.nk[0] = nk[0];
-    rec.nk[1] = nk[1];
-    rec.nk[2] = nk[2];
-    rec.flags = merged ? 1u : 0u;
-    rec.pad[0] = rec.pad[1] = rec.pad[2] = 0;
-    *R = rec;
+  // the stored constants in Montgomery form, converted by lanes 0..8 at once
+  V cv = vone;
+  switch (lane) {
+    case 0: cv = mbase; break;
+    case 1: cv = bn; break;
+    case 2: cv = gQ_inv; break;
+    case 3: cv = g1; break;
+    case 4: cv = gT0; break;
+    case 5: cv = gT1; break;
+    case 6: cv = merged ? bT1 : bT0; break;
+    case 7: cv = bT1; break;
+    default: break;
+  }
+  const uint64_t cm = ar.mont(cv);
+  PrimeRec rec;
+  rec.p = p;
+  rec.q = q;
+  rec.one = one;
+  rec.mbase = shfl64(cm, 0);
+  rec.bn = shfl64(cm, 1);
+  rec.g[0] = shfl64(cm, 2);
+  rec.g[1] = shfl64(cm, 3);
+  rec.gT[0] = shfl64(cm, 4);
+  rec.gT[1] = shfl64(cm, 5);
+  rec.bT[0] = shfl64(cm, 6);
+  rec.bT[1] = shfl64(cm, 7);
+  rec.nk[0] = nk[0];
+  rec.nk[1] = nk[1];
+  rec.nk[2] = nk[2];
+  rec.flags = merged ? 1u : 0u;
+  rec.pad[0] = rec.pad[1] = rec.pad[2] = 0;
+  if (lane == 0) *R = rec;
+}
+
+__global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
+  const SegJob& J = jobs[blockIdx.y];
+  Counters* __restrict__ cnt = J.cnt;
+  const uint32_t cap = J.cap;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t pi = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t ngen = J.nprimes;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicAdd((unsigned long long*)&cnt->primecount, (unsigned long long)ngen);
+    atomicMax((unsigned long long*)&cnt->max_seg_primes, (unsigned long long)ngen);
+    if (ngen > cap) atomicAdd((unsigned long long*)&cnt->seg_overflow, 1ull);
+  }
+  const uint32_t np = ngen < cap ? ngen : cap;
+  if (pi >= np) return;
+  const uint64_t p = J.primes[pi];
+  const uint64_t q = mont_inv(p);
+  const uint64_t one = r_mod(p);
+  if ((p & 1) && p < kFpLimit) {
+    FpArith ar;
+    ar.p = (double)p;
+    ar.pinv = 1.0 / ar.p;
+    ar.R = (double)one;
+    setup_prime(ar, J, p, q, one, pi, lane);
+  } else {
+    MontArith ar;
+    ar.p = p;
+    ar.q = q;
+    ar.one_ = one;
+    uint64_t r2 = madd(madd(one, one, p), madd(one, one, p), p);
+#pragma unroll
+    for (int i = 0; i < 5; i++) r2 = mmul(r2, r2, p, q);  // 4^(2^5) = 2^64: Montgomery form of R
+    ar.r2 = r2;
+    setup_prime(ar, J, p, q, one, pi, lane);
   }
 }
 

@@ -139,6 +139,23 @@ def test_sieve_k_above_2_31(ctx, k31):
     _compare(base, ks, cs, nmin, nmax, wps, bm, gpu, orc, k31=k31)
 
 
+def test_sieve_k_2_31_compat_wrap(ctx):
+    """k = 2^31 under the reference's signed-int encoding: the entry is -2^31
+    whatever c is, and its magnitude reaches the setup sign-extended to 64
+    bits (setup.cl:143).  Base 3 (even k), p below and above 2^51."""
+    base, nmin, nmax = 3, 20000, 60000
+    ks = [2**31, 2**31, 2**31 + 2, 2**32 - 2]
+    cs = [1, -1, 1, -1]
+    bm, wps = full_bitmap(len(ks), nmin, nmax)
+    prm = R.reference_params(nmin, nmax, len(ks), 65536, ctx.compute_units)
+    prm.k31_compat = 1
+    for pmin, pmax in ((10**8, 10**8 + 10**6), (2**51 - 10**5, 2**51 + 10**5)):
+        print("k = 2^31 compat case", pmin, pmax, flush=True)
+        gpu = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, prm)
+        orc = oracle_run(base, ks, cs, nmin, nmax, bm, pmin, pmax, k31=True)
+        _compare(base, ks, cs, nmin, nmax, wps, bm, gpu, orc, k31=True)
+
+
 OVERFLOW_CASES = [c for c in CASES if c[0] in ("b2_k10_1e9", "b2_k20_2e40", "b7_k2_2e63", "b2_k12_2e51")]
 
 
@@ -236,3 +253,76 @@ def test_batch_of_uneven_jobs_matches_oracle(ctx):
             assert cnt["dups"] == ocnt["dups"]
         assert np.array_equal(gbm, obm)
         assert {(int(f["kidx"]), int(f["n"])) for f in facs} == {(k, n) for (_, n, k) in ofac}
+
+
+def _high_p_data(seed=63, K=100, nmin=10**6, nmax=2**24 - 1, and_words=5):
+    """The high-p production shape: base 5, K sequences with k in [2, 2^32)
+    coprime to 5 (even and odd k), c alternating -1, +1, n in [10^6, 2^24),
+    and a bitmap keeping about 1/32 of the terms (the AND of five random
+    words per word), the share a pre-sieve to 2^20 leaves."""
+    rng = np.random.default_rng(seed)
+    ks = []
+    while len(ks) < K:
+        k = int(rng.integers(2, 2**32))
+        if k % 5 == 0 or k in ks:
+            continue
+        ks.append(k)
+    cs = [1 if i % 2 else -1 for i in range(K)]
+    wps = (nmax - nmin + 1 + 31) // 32
+    bm = rng.integers(0, 2**32, size=K * wps, dtype=np.uint32)
+    for _ in range(and_words - 1):
+        bm &= rng.integers(0, 2**32, size=K * wps, dtype=np.uint32)
+    tail = (nmax - nmin + 1) % 32
+    if tail:  # bits past nmax stay clear
+        bm.reshape(K, wps)[:, -1] &= np.uint32((1 << tail) - 1)
+    return ks, cs, nmin, nmax, bm, wps
+
+
+def test_high_p_production_shape_batched_and_tuned(ctx):
+    """b = 5, 100 sequences, n in [10^6, 2^24), p from 2^63 (integer
+    Montgomery path near the top of the 64-bit range): the reference-shaped
+    sieve equals the oracle counter for counter, and the batched path (two
+    views of the job in one batch) with the cost model's configuration (the
+    largest merged table when no model is committed) leaves the same bitmap."""
+    base = 5
+    ks, cs, nmin, nmax, bm, wps = _high_p_data()
+    K = len(ks)
+    pmin, pmax = 2**63, 2**63 + 2 * 10**4
+    print("high-p case", base, K, nmin, nmax, pmin, pmax, flush=True)
+    orc = oracle_run(base, ks, cs, nmin, nmax, bm, pmin, pmax, nthreads=8)
+    prm = R.reference_params(nmin, nmax, K, 65536, ctx.compute_units)
+    gpu = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, prm)
+    gcnt, gfac, gbm = gpu
+    ocnt, ofac, obm = orc
+    assert gcnt["primecount"] == ocnt["primecount"] > 400
+    assert gcnt["skipped"] == ocnt["skipped"]
+    assert list(gcnt["lists"]) == [ocnt["n_full"], ocnt["n_even"], ocnt["n_odd"]]
+    assert gcnt["order_lists"] == ocnt["order_lists"]
+    assert gcnt["claims"] == ocnt["claims"] and gcnt["dups"] == ocnt["dups"]
+    assert np.array_equal(gbm, obm)
+
+    cands = ctx.tune_candidates(nmin, nmax, K)
+    try:
+        from rssieve_amd import tuning
+        model = tuning.load_model(ctx)
+        x = tuning.features(base, K, int(np.bitwise_count(bm).sum()), nmax - nmin + 1, 63.0)[None, :]
+        idx = int(model.choose(x, np.array([[c.threads != 0 for c in cands]]))[0])
+    except (OSError, KeyError):
+        idx = max((i for i, c in enumerate(cands) if c.algo == R.ALGO_MERGED and c.threads), key=lambda i: cands[i].Q)
+    print("tuned candidate", idx, cands[idx].as_dict(), flush=True)
+    job = ctx.job(base, ks, cs, nmin, nmax, bm, wps, cands[idx])
+    view = job.view()
+    mid = pmin + (pmax - pmin) // 3
+    R.sieve_batch([job, view], [pmin, mid], [mid, pmax])
+    cnt = job.counters().as_dict()
+    facs = job.drain()
+    tbm = job.bitmap()
+    view.close()
+    job.close()
+    assert cnt["primecount"] == ocnt["primecount"]
+    assert cnt["skipped"] == ocnt["skipped"]
+    assert cnt["claims"] == ocnt["claims"]
+    assert np.array_equal(tbm, obm)
+    for f in facs:
+        i = int(f["kidx"])
+        assert R.verify_factor(int(f["p"]), ks[i], int(f["n"]), cs[i], base) == 1
