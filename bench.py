@@ -42,6 +42,7 @@ P0 = 2**50
 PRANGE = 2**38
 SLICE = 2**28
 K = 100
+ALGO_NAMES = {0: "three lists", 1: "merged", 2: "residue classes"}
 SEED = 2026
 PRESIEVE = 2**20
 # Every modular product of the BSGS kernel on this p-range (p < 2^51) is the
@@ -294,7 +295,7 @@ def run_tuned(ctx, data, steps, warmup, batch):
     return {"primes": c1.primecount - c0.primecount, "seconds": t1 - t0, "bitmap": bm,
             "roof": roofline(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % cfg.threads, "tuned"),
             "busy": bsgs_s / (t1 - t0), "claims": c1.claims, "dups": c1.dups,
-            "cfg": {"algo": "merged" if cfg.algo == R.ALGO_MERGED else "three lists", "Q": cfg.Q,
+            "cfg": {"algo": ALGO_NAMES[cfg.algo], "Q": cfg.Q,
                     "hsize": cfg.hsize, "threads": cfg.threads, "seg_primes": cfg.seg_primes, "batch": batch,
                     "chosen_by": how, "candidate": idx},
             "log": log}

@@ -38,7 +38,10 @@ enum {
 /* BSGS algorithm of a job. */
 enum {
   RS_ALGO_THREE_LISTS = 0, /* reference shape: full / even / odd k lists, one table each (giant.cl) */
-  RS_ALGO_MERGED = 1       /* one table of even exponents shared by all classes */
+  RS_ALGO_MERGED = 1,      /* one table of even exponents shared by all classes */
+  RS_ALGO_CLASSES = 2      /* one table per prime of stride S: the class of n mod S each target's
+                              power-residue character fixes (S | 2 * gcd(p - 1, 240)); Q is the
+                              largest table, in entries; the eliminated terms are the same */
 };
 
 typedef struct rs_ctx rs_ctx;
@@ -72,7 +75,8 @@ typedef struct {
  * choice (cl_sieve.cpp:693-735 and 885-938). */
 typedef struct {
   int32_t algo;            /* RS_ALGO_* */
-  uint32_t Q;              /* baby steps of the full list; parity lists use QQ = 2Q */
+  uint32_t Q;              /* baby steps of the full list; parity lists use QQ = 2Q (merged: the
+                              exponent span 2 * entries; residue classes: the largest table's entries) */
   uint32_t hsize;          /* LDS filter words, power of 2, >= 64 and >= the baby-step entries */
   uint32_t threads;        /* BSGS workgroup size: 128, 256, 512 or 1024 */
   uint32_t seg_primes;     /* target primes per segment (reference: 256 * compute units) */

@@ -8,6 +8,7 @@ missing.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -17,7 +18,7 @@ CLI_PATH = os.path.join(HERE, "rssieve")
 MAX_SEQ = 100
 
 RS_OK, RS_E_ARG, RS_E_HIP, RS_E_NOMEM, RS_E_OVERFLOW, RS_E_IO, RS_E_VERIFY = 0, -1, -2, -3, -4, -5, -6
-ALGO_THREE_LISTS, ALGO_MERGED = 0, 1
+ALGO_THREE_LISTS, ALGO_MERGED, ALGO_CLASSES = 0, 1, 2
 
 
 class RSError(RuntimeError):
@@ -297,6 +298,9 @@ class Context:
         h = C.c_void_p()
         _check(lib().rs_open(device, C.byref(h)))
         self._h = h
+        # jobs and MLPs live on the context's stream and device memory: they
+        # are closed before it
+        self._children = weakref.WeakSet()
         cu, lds = C.c_uint32(), C.c_uint32()
         name = C.create_string_buffer(256)
         _check(lib().rs_device_info(h, C.byref(cu), C.byref(lds), name, 256))
@@ -314,11 +318,13 @@ class Context:
         return out[:min(cnt.value, cap)]
 
     def tune_candidates(self, nmin, nmax, kcount):
-        cap = 64
+        cap = 128
         arr = (Params * cap)()
         n = C.c_int()
         _check(lib().rs_tune_candidates(self._h, nmin, nmax, kcount, arr, cap, C.byref(n)))
-        return [arr[i] for i in range(min(n.value, cap))]
+        if n.value > cap:
+            raise RSError("rs_tune_candidates: %d candidates, room for %d" % (n.value, cap))
+        return [arr[i] for i in range(n.value)]
 
     def mlp(self, n_in, n_hidden, n_out, seed=1):
         return MLP(self, n_in, n_hidden, n_out, seed)
@@ -328,6 +334,8 @@ class Context:
 
     def close(self):
         if self._h:
+            for ch in list(self._children):
+                ch.close()
             lib().rs_close(self._h)
             self._h = None
 
@@ -354,6 +362,7 @@ class Job:
         _check(lib().rs_job_create(ctx._h, base, len(self.k), _u32p(self.k), _i32p(self.c), nmin, nmax,
                                    _u32p(bitmap), words_per_seq, C.byref(params), C.byref(h)))
         self._h = h
+        ctx._children.add(self)
 
     def sieve(self, pmin, pmax):
         _check(lib().rs_sieve(self._h, pmin, pmax))
@@ -397,6 +406,7 @@ class Job:
         h = C.c_void_p()
         _check(lib().rs_job_view(self._h, C.byref(h)))
         v._h = h
+        self.ctx._children.add(v)
         return v
 
     def reconfigure(self, params):
@@ -446,6 +456,7 @@ class MLP:
         h = C.c_void_p()
         _check(lib().rs_mlp_create(ctx._h, n_in, n_hidden, n_out, seed, C.byref(h)))
         self._h = h
+        ctx._children.add(self)
         self.n_in, self.H, self.n_out = n_in, n_hidden, n_out
         n = C.c_uint64()
         _check(lib().rs_mlp_num_params(h, C.byref(n)))

@@ -252,13 +252,15 @@ static bool pow2(uint32_t x) { return x && !(x & (x - 1)); }
 
 // Checks of a BSGS configuration against a job shape and the device.
 static int check_params(const rs_ctx* ctx, const rs_params* prm, const char* who) {
-  if (prm->algo != RS_ALGO_THREE_LISTS && prm->algo != RS_ALGO_MERGED)
+  if (prm->algo != RS_ALGO_THREE_LISTS && prm->algo != RS_ALGO_MERGED && prm->algo != RS_ALGO_CLASSES)
     return fail(RS_E_ARG, "%s: unknown algorithm %d", who, prm->algo);
   if (prm->algo == RS_ALGO_MERGED && (prm->Q & 1))
     return fail(RS_E_ARG, "%s: the merged table needs an even exponent span Q (got %u)", who, prm->Q);
   if (!(prm->threads == 128 || prm->threads == 256 || prm->threads == 512 || prm->threads == 1024))
     return fail(RS_E_ARG, "%s: threads %u not in {128,256,512,1024}", who, prm->threads);
   const uint32_t entries = prm->algo == RS_ALGO_MERGED ? prm->Q / 2 : prm->Q;  // baby steps in one table
+  if (prm->algo == RS_ALGO_CLASSES && (prm->Q < 64 || prm->Q % 64))
+    return fail(RS_E_ARG, "%s: residue-class tables need a multiple of 64 entries (got %u)", who, prm->Q);
   if (prm->Q < 16 || entries > 16384 || !pow2(prm->hsize) || prm->hsize < 64 || prm->hsize < entries)
     return fail(RS_E_ARG, "%s: need at most 16384 baby-step entries and a power-of-two filter of at least 64 "
                 "words and at least one word per entry (Q=%u entries=%u hsize=%u)", who, prm->Q, entries, prm->hsize);
@@ -457,7 +459,7 @@ static int record_event(rs_job* j, hipEvent_t* out) {
 
 // Baby-step entries of the largest list of a job's BSGS shape.
 static uint32_t job_max_entries(const JobDev& h) {
-  return h.algo == 1 ? h.Q / 2 : std::max(h.Q, h.QQ / 2);
+  return h.algo == 1 ? h.Q / 2 : h.algo == 2 ? h.Q : std::max(h.Q, h.QQ / 2);
 }
 
 // Size the overflow-recovery scratch for 'blocks' workgroups and lists of up
@@ -834,7 +836,8 @@ extern "C" int rs_job_reconfigure(rs_job* j, const rs_params* prm) {
 // split; then the merged table at 1024..16384 entries, filter words 1, 2 or 4
 // per entry (where the LDS holds them), and 256, 512 or 1024 threads per
 // workgroup; then merged spans sized so the giant steps fill F = 1..4 whole
-// rounds of the workgroup (the Q of these depends on the n-range).
+// rounds of the workgroup (the Q of these depends on the n-range); then the
+// residue-class tables.
 extern "C" int rs_tune_candidates(rs_ctx* ctx, uint32_t nmin, uint32_t nmax, int32_t kcount, rs_params* out, int cap,
                                   int* n) {
   if (!ctx || !out || !n || cap < 1) return fail(RS_E_ARG, "rs_tune_candidates: bad args");
@@ -879,6 +882,21 @@ extern "C" int rs_tune_candidates(rs_ctx* ctx, uint32_t nmin, uint32_t nmax, int
         if (e < 512 || e > 16384 || (L + Q - 1) / Q != (uint64_t)t * F ||
             check_params(ctx, &p, "rs_tune_candidates") != RS_OK)
           p.threads = 0;
+        if (m < cap) out[m] = p;
+        m++;
+      }
+  // Residue-class tables of at most 1024..16384 entries, filter load 1 or
+  // 1/2, 256 or 512 threads (20 fixed slots; threads = 0 where the LDS
+  // cannot hold them).
+  for (uint32_t e : entries)
+    for (uint32_t l = 1; l <= 2; l++)
+      for (uint32_t t : {256u, 512u}) {
+        rs_params p = ref;
+        p.algo = RS_ALGO_CLASSES;
+        p.Q = e;
+        p.hsize = e * l;
+        p.threads = t;
+        if (check_params(ctx, &p, "rs_tune_candidates") != RS_OK) p.threads = 0;
         if (m < cap) out[m] = p;
         m++;
       }

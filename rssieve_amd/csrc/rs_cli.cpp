@@ -416,7 +416,7 @@ int sieve(Options o) {
   if (o.merged && st.pmax - st.p > 4096) {
     // -t: time every candidate configuration on a short sample at the start of
     // the range (the job's bitmap and counters are restored afterwards)
-    std::vector<rs_params> cand(64);
+    std::vector<rs_params> cand(128);
     int nc = 0;
     if (rs_tune_candidates(r.ctx, nmin, nmax, kcount, cand.data(), (int)cand.size(), &nc)) die_lib("rs_tune_candidates");
     nc = std::min<int>(nc, (int)cand.size());
@@ -433,7 +433,7 @@ int sieve(Options o) {
     int best = 0;
     if (rs_autotune(r.job, st.p, width, cand.data(), nc, secs.data(), pr.data(), &best)) die_lib("rs_autotune");
     prm = cand[best];
-    printf("Autotuned: candidate %d of %d (%s)\n", best, nc, prm.algo == RS_ALGO_MERGED ? "merged table" : "three lists");
+    printf("Autotuned: candidate %d of %d (%s)\n", best, nc, prm.algo == RS_ALGO_CLASSES ? "residue-class tables" : prm.algo == RS_ALGO_MERGED ? "merged table" : "three lists");
   }
   uint32_t M, QQ, MM;
   rs_params_derive(&prm, nmin, nmax, &M, &QQ, &MM);

@@ -17,7 +17,7 @@ struct JobDev {
   uint64_t ak[RS_MAXK];       // |k| fed to the Montgomery setup (setup.cl:143)
   int32_t cpos[RS_MAXK];      // 1 if the target is -k^-1 (c = +1), setup.cl:188
   // BSGS shape
-  uint32_t algo;              // 0: three lists (reference shape), 1: one merged table
+  uint32_t algo;              // 0: three lists (reference shape), 1: one merged table, 2: residue-class tables
   uint32_t Q, M, QQ, MM;      // baby steps / giant steps, full and parity lists
   uint32_t Qref;              // the reference's Q for this n-range (the setup invertibility test)
   uint32_t hsize, hmask;      // LDS hash slots (power of 2)
@@ -36,16 +36,20 @@ struct PrimeRec {
   uint64_t gT[2];             // g[c]^threads
   uint64_t bT[2];             // b^threads and b^(2*threads): baby stride per round
   uint32_t nk[RS_NCLASS];     // k count per class: full, even, odd (0 = skipped)
-  uint32_t flags;             // reserved
-  uint64_t pad[3];
+  uint32_t flags;             // the job's algo when written
+  // residue-class tables (algo 2): baby values b^(NMIN + S*e), e < E; M giant
+  // steps of b^(-S*E); every target carries its n mod S as a shift
+  uint32_t S, E, M, pad32;
+  uint64_t pad;
 };
 static_assert(sizeof(PrimeRec) == 128, "PrimeRec layout");
 
 // Per (prime, k) entry, grouped by class: full, then even, then odd.
 struct KRec {
-  uint64_t hadj;              // target b^n: k^-1 (c=-1) or -k^-1 (c=+1), Montgomery
+  uint64_t hadj;              // target b^n: k^-1 (c=-1) or -k^-1 (c=+1), Montgomery (times b^-d)
   uint32_t kidx;              // sequence index
-  uint32_t cls;               // 1 full, 2 even, 3 odd (the reference's parity code)
+  uint32_t cls;               // three lists: 1 full, 2 even, 3 odd (the reference's parity code);
+                              // merged and residue-class tables: the shift d of the target
 };
 
 // Factor record handed to the host (p, n, sequence index).

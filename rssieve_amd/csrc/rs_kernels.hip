@@ -269,6 +269,27 @@ __device__ __forceinline__ void fp_powmod_lg2(double x0, double x1, uint64_t exp
 // d of the power-residue tests (setup.cl:22-48)
 __constant__ uint32_t g_prd[12] = {3, 4, 5, 7, 8, 9, 11, 13, 17, 19, 23, 29};
 
+// Residue classes of n (algo 2).  With L = gcd(p - 1, 240) and z =
+// b^((p-1)/L), every solution of b^n = h (mod p) satisfies z^n = y with y =
+// h^((p-1)/L).  The powers z^i, i < s = ord(z), are distinct, so y = z^x
+// fixes n = x (mod s), and a y that is no power of z leaves no n at all.
+// This is algebra mod p, true for a composite 2-PRP as well: the reference's
+// tests (parity classes, power residues, setup.cl:59-87) still decide which
+// k are searched, and the classes only narrow the n they are searched on, so
+// the solutions found are the same.  For a prime p with b a quadratic
+// non-residue s is even and fixes the parity the reference's class gives.
+constexpr uint32_t kZL = 240;                 // L divides 240 = 16 * 3 * 5
+constexpr uint32_t kZV = 256;                 // z^i table entries (>= kZL + 1)
+constexpr uint32_t kZH = 512;                 // hash slots over them (power of two)
+constexpr uint64_t kClassMinP = 1024;         // smaller p keep the merged table's parity split
+
+// LDS operations of one wavefront become visible to its other lanes
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // The residue arithmetic of the setup, in two representations with one
 // interface.  Every value the setup computes is a canonical residue, so both
 // give the same stored records bit for bit; only the cost differs.
@@ -293,8 +314,11 @@ struct MontArith {
   __device__ V base_pow(V mb, uint32_t e, uint32_t b) const { return basepow_sm(mb, e, b, p, q, one_); }
   __device__ V pow_lg(V x, uint64_t e) const { return powmod_lg(x, e, p, q); }
   __device__ V pow_sm(V x, uint32_t e) const { return powmod_sm(x, e, p, q, one_); }
+  // x^e for any 64-bit e (powmod_lg is exact from e = 2 on)
+  __device__ V pow_u64(V x, uint64_t e) const { return e <= 1 ? (e ? x : one_) : powmod_lg(x, e, p, q); }
   __device__ void pow_lg2(V x0, V x1, uint64_t e, V* r0, V* r1) const { powmod_lg2(x0, x1, e, p, q, r0, r1); }
   __device__ uint64_t mont(V x) const { return x; }
+  __device__ uint64_t bits(V x) const { return x; }  // a canonical integer key
 };
 
 struct FpArith {
@@ -314,15 +338,17 @@ struct FpArith {
   __device__ V base_pow(V mb, uint32_t e, uint32_t) const { return fp_powmod_sm(mb, e, p, pinv); }
   __device__ V pow_lg(V x, uint64_t e) const { return fp_powmod_lg(x, e, p, pinv); }
   __device__ V pow_sm(V x, uint32_t e) const { return fp_powmod_sm(x, e, p, pinv); }
+  __device__ V pow_u64(V x, uint64_t e) const { return e <= 1 ? (e ? x : 1.0) : fp_powmod_lg(x, e, p, pinv); }
   __device__ void pow_lg2(V x0, V x1, uint64_t e, V* r0, V* r1) const { fp_powmod_lg2(x0, x1, e, p, pinv, r0, r1); }
   __device__ uint64_t mont(V x) const { return fp_to_u64(fp_mulmod(x, R, p, pinv)); }
+  __device__ uint64_t bits(V x) const { return fp_to_u64(x); }
 };
 
 // The setup of prime p = primes[pi] by one wavefront (setup.cl:89-209 and
-// sort.cl:11-104).
+// sort.cl:11-104); zv and zh are the wavefront's LDS for the z^i table.
 template <class A>
 __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64_t p, uint64_t q, uint64_t one,
-                                            uint32_t pi, uint32_t lane) {
+                                            uint32_t pi, uint32_t lane, uint64_t* zv, uint32_t* zh) {
   using V = typename A::V;
   const JobDev* __restrict__ job = J.job;
   PrimeRec* __restrict__ R = J.recs + pi;
@@ -330,6 +356,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   const uint32_t base = job->base;
   const int K = job->kcount;
   const uint32_t T = J.threads;
+  const uint32_t algo = job->algo;
   const V vone = ar.one(), nmo = ar.neg_one();
   const V mbase = ar.base(base);
 
@@ -368,11 +395,22 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   const V AB = ar.mul(A0, B0);
   const V total = ar.mul(AB, gQ);  // prefix[KCOUNT] of setup.cl:147-151
 
+  // residue classes: L = gcd(p - 1, 240)
+  const uint64_t pm = p - 1;
+  const bool split = algo == 2 && p > kClassMinP;
+  uint32_t L = 2;
+  uint64_t exL = 0;
+  if (split) {
+    const uint32_t v2 = (uint32_t)__builtin_ctzll(pm);
+    L = (1u << (v2 < 4 ? v2 : 4)) * (pm % 3 == 0 ? 3u : 1u) * (pm % 5 == 0 ? 5u : 1u);
+    exL = pm / L;
+  }
+
   // Lane-parallel exponentiations, one code path (no serialized branches):
   //  lane 0: Fermat inverse total^(p-2)          (setup.cl:154)
   //  lanes 1..12: base^((p-1)/d) for d | p-1       (setup.cl:22-48)
   //  lane 13: base^((p-1)/2)                       (setup.cl:51-53)
-  const uint64_t pm = p - 1;
+  //  lane 14: z = base^((p-1)/L) of the residue classes
   uint64_t ex = 0;
   V x = mbase;
   bool dvalid = false;
@@ -386,13 +424,16 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     if (!dvalid) ex = 0;
   } else if (lane == 13) {
     ex = pm >> 1;
+  } else if (lane == 14) {
+    ex = exL;
   }
   V r = vone;
   if (ex) r = ar.pow_lg(x, ex);
   const V inv_total = shfl64(r, 0);
   const bool invertible = ar.mul(total, inv_total) == vone;  // setup.cl:156-162
-  const uint64_t dmask = __ballot(dvalid && r == vone) >> 1;  // passing d tests, bit i = g_prd[i]
+  const uint64_t dmask = __ballot(lane <= 12 && dvalid && r == vone) >> 1;  // passing d tests, bit i = g_prd[i]
   const V rg = shfl64(r, 13);
+  const V zr = shfl64(r, 14);
   const int resg = (rg == vone) - (rg == nmo);
 
   if (!invertible) {
@@ -430,9 +471,19 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     if (t0 != 1) cls0 = 0;
     if (t1 != 1) cls1 = 0;
   }
+  // the quadratic character h^((p-1)/2); with the residue classes it is
+  // y^(L/2) for the y = h^((p-1)/L) the classes need anyway (same value)
+  V y0 = vone, y1 = vone;
   {
     int t0, t1;
-    pow_sign2(pm >> 1, &t0, &t1);
+    if (split) {
+      ar.pow_lg2(h0, h1, exL, &y0, &y1);
+      const V a0 = ar.pow_u64(y0, L / 2), a1 = ar.pow_u64(y1, L / 2);
+      t0 = (a0 == vone) - (a0 == nmo);
+      t1 = (a1 == vone) - (a1 == nmo);
+    } else {
+      pow_sign2(pm >> 1, &t0, &t1);
+    }
     if (cls0) {
       if (resg == -1) cls0 = (t0 == 1) ? 2 : (t0 == -1) ? 3 : 0;
       else if (resg == 1 && t0 == -1) cls0 = 0;
@@ -450,11 +501,10 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     }
     return;
   }
-  const bool merged = job->algo == 1;
+  const bool merged = algo != 0;
 
-  // Constants of the BSGS kernel, lane-parallel in two rounds:
+  // Constants of the BSGS kernel, lane-parallel:
   //  lanes 0, 3, 4, 5: b^NMIN, b^T, b^(2T), b^(Qref-1)  ->  b^-1 = b^(Qref-1) * b^-Qref
-  //  lanes 0, 1, 2:    b^-Q, b^-QT, b^-2QT (giant step of the table span Q)
   V y = vone;
   uint32_t ye = 0;
   if (lane == 0) ye = job->nmin;
@@ -464,62 +514,94 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   if (lane == 0 || (lane >= 3 && lane <= 5)) y = ar.pow_sm(mbase, ye);
   const V bn = shfl64(y, 0), bT0 = shfl64(y, 3), bT1 = shfl64(y, 4);
   const V binv = ar.mul(shfl64(y, 5), gQref_inv);
-  ye = 0;
-  if (lane == 0) ye = job->Q;
-  else if (lane == 1) ye = job->Q * T;
-  else if (lane == 2) ye = 2 * job->Q * T;
-  y = vone;
-  if (lane < 3) y = ar.pow_sm(binv, ye);
-  const V gQ_inv = shfl64(y, 0), gT0 = shfl64(y, 1), gT1 = shfl64(y, 2);
-  const V g1 = ar.mul(gQ_inv, gQ_inv);
 
-  uint32_t nk[3];
+  uint32_t nk[3] = {0, 0, 0};
+  uint32_t cS = 0, cE = 0, cM = 0;  // residue-class shape of this prime
+  V g0v, g1v, gT0v, gT1v, bTa, bTb;
   KRec* kr = J.krecs + (size_t)pi * (merged ? 2 * K : K);
-  if (!merged) {
-    // compaction by class: full, even, odd (the reference's sort.cl:40-71)
-    const uint64_t hm0 = ar.mont(h0), hm1 = ar.mont(h1);
-    uint32_t off = 0;
-#pragma unroll
-    for (int c = 1; c <= 3; c++) {
-      const uint64_t m0 = __ballot(cls0 == c), m1 = __ballot(cls1 == c);
-      const uint32_t c0 = __popcll(m0), c1 = __popcll(m1);
-      if (cls0 == c) {
-        KRec e;
-        e.hadj = hm0;
-        e.kidx = lane;
-        e.cls = c;
-        kr[off + lane_rank(m0)] = e;
-      }
-      if (cls1 == c) {
-        KRec e;
-        e.hadj = hm1;
-        e.kidx = lane + 64;
-        e.cls = c;
-        kr[off + c0 + lane_rank(m1)] = e;
-      }
-      nk[c - 1] = c0 + c1;
-      off += c0 + c1;
+  if (split) {
+    // the table of z^i, 0 <= i <= L, and s = ord(z) (0: z^i != 1 for 0 < i <= L,
+    // possible only for a composite p; then the classes are the parity split)
+    for (uint32_t h = lane; h < kZH; h += 64) zh[h] = 0u;
+    V cur = ar.pow_u64(zr, lane);
+    const V z64 = ar.mul(shfl64(cur, 63), zr);
+    uint32_t s = 0;
+    for (uint32_t j = 0; 64 * j <= L; j++) {
+      const uint32_t idx = 64 * j + lane;
+      const bool in = idx <= L;
+      if (in) zv[idx] = ar.bits(cur);
+      const uint64_t m1 = __ballot(in && idx > 0 && cur == vone);
+      if (!s && m1) s = 64 * j + (uint32_t)__ffsll((long long)m1) - 1;
+      cur = ar.mul(cur, z64);
     }
-  } else {
-    // One list against a table of even exponent offsets: a target with
-    // shift d searches b^(n - d) = hadj * b^-d.  Full-range k get d = 0 and
-    // d = 1, even-only k d = 0, odd-only k d = 1; cls holds d.
+    wave_sync();
+    for (uint32_t idx = lane; idx < s; idx += 64) {
+      uint32_t h = (uint32_t)zv[idx] & (kZH - 1);
+      while (atomicCAS(&zh[h], 0u, idx + 1) != 0u) h = (h + 1) & (kZH - 1);
+    }
+    wave_sync();
+    // discrete log of y to the base z (index in the table), or -1
+    auto dlog = [&](V yv) -> int {
+      const uint64_t key = ar.bits(yv);
+      for (uint32_t h = (uint32_t)key & (kZH - 1);; h = (h + 1) & (kZH - 1)) {
+        const uint32_t sl = zh[h];
+        if (!sl) return -1;
+        if (zv[sl - 1] == key) return (int)sl - 1;
+      }
+    };
+    // Table stride S: s when s is even (it fixes the parity) or no k has a
+    // parity class here, else 2s (CRT of the class mod s and the parity);
+    // 2 without a usable s.  Each target gets one or two residues n mod S.
+    const bool anyPar = __ballot(cls0 >= 2 || cls1 >= 2) != 0;
+    const uint32_t S = s == 0 ? 2u : ((s & 1) && anyPar) ? 2 * s : s;
+    auto classes = [&](int cls, V yv, uint32_t (&rr)[2]) -> int {
+      if (!cls) return 0;
+      if (s == 0) {
+        if (cls == 1) {
+          rr[0] = 0;
+          rr[1] = 1;
+          return 2;
+        }
+        rr[0] = (uint32_t)(cls - 2);
+        return 1;
+      }
+      const int xl = dlog(yv);
+      if (xl < 0) return 0;
+      const uint32_t ux = (uint32_t)xl;
+      if (S == s) {
+        if (cls >= 2 && (ux & 1u) != (uint32_t)(cls - 2)) return 0;  // s even: the parity contradicts
+        rr[0] = ux;
+        return 1;
+      }
+      if (cls == 1) {
+        rr[0] = ux;
+        rr[1] = ux + s;
+        return 2;
+      }
+      rr[0] = ((ux & 1u) == (uint32_t)(cls - 2)) ? ux : ux + s;
+      return 1;
+    };
+    uint32_t ra[2] = {0, 0}, rb[2] = {0, 0};
+    const int na = classes(cls0, y0, ra), nb = classes(cls1, y1, rb);
+    const uint32_t nmod = job->nmin % S;
     uint32_t off = 0;
 #pragma unroll
-    for (int d = 0; d <= 1; d++) {
-      const bool s0 = cls0 == 1 || cls0 == 2 + d, s1 = cls1 == 1 || cls1 == 2 + d;
-      const uint64_t m0 = __ballot(s0), m1 = __ballot(s1);
+    for (int a = 0; a < 2; a++) {
+      const bool va = na > a, vb = nb > a;
+      const uint64_t m0 = __ballot(va), m1 = __ballot(vb);
       const uint32_t c0 = __popcll(m0), c1 = __popcll(m1);
-      if (s0) {
+      if (va) {
+        const uint32_t d = (ra[a] + S - nmod) % S;
         KRec e;
-        e.hadj = ar.mont(d ? ar.mul(h0, binv) : h0);
+        e.hadj = ar.mont(d ? ar.mul(h0, ar.pow_u64(binv, d)) : h0);
         e.kidx = lane;
         e.cls = d;
         kr[off + lane_rank(m0)] = e;
       }
-      if (s1) {
+      if (vb) {
+        const uint32_t d = (rb[a] + S - nmod) % S;
         KRec e;
-        e.hadj = ar.mont(d ? ar.mul(h1, binv) : h1);
+        e.hadj = ar.mont(d ? ar.mul(h1, ar.pow_u64(binv, d)) : h1);
         e.kidx = lane + 64;
         e.cls = d;
         kr[off + c0 + lane_rank(m1)] = e;
@@ -527,19 +609,140 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
       off += c0 + c1;
     }
     nk[0] = off;
-    nk[1] = nk[2] = 0;
+    // Table length: E* = sqrt(span * targets / (2S)) balances the baby
+    // steps (set, then matched) against the giant-step probes.  The giant
+    // steps are then rounded to F whole rounds of the workgroup (M = F*T,
+    // E = span / (S*M) rounded up), which keeps every probe in the
+    // kernel's pipelined loop; below half a round the balanced shape is
+    // kept.  E is at most the job's Q entries, and a capped table takes as
+    // many giant steps as it needs.
+    const uint32_t span = job->nmax - job->nmin + 1;
+    const double estar = sqrt((double)span * (double)(off + 1) / (2.0 * (double)S));
+    const double mstar = (double)span / ((double)S * estar);
+    uint64_t E, M;
+    if (mstar < 0.5 * (double)T) {
+      E = (uint64_t)ceil(estar);
+      M = ((uint64_t)span + (uint64_t)S * E - 1) / ((uint64_t)S * E);
+    } else {
+      const uint64_t F = (uint64_t)rint(mstar / (double)T);
+      E = ((uint64_t)span + (uint64_t)S * F * T - 1) / ((uint64_t)S * F * T);
+      M = F * T;
+    }
+    if (E > job->Q) {
+      E = job->Q;
+      M = ((uint64_t)span + (uint64_t)S * E - 1) / ((uint64_t)S * E);
+    }
+    const uint64_t SE = (uint64_t)S * E;
+    cS = S;
+    cE = (uint32_t)E;
+    cM = (uint32_t)M;
+    // giant step b^-(S*E), its T-th power, and the baby stride b^(S*T)
+    uint64_t ye2 = 0;
+    V yb2 = binv;
+    if (lane == 0) ye2 = SE;
+    else if (lane == 1) ye2 = SE * T;
+    else if (lane == 2) {
+      ye2 = (uint64_t)S * T;
+      yb2 = mbase;
+    }
+    V y2 = vone;
+    if (lane < 3) y2 = ar.pow_u64(yb2, ye2);
+    g0v = g1v = shfl64(y2, 0);
+    gT0v = gT1v = shfl64(y2, 1);
+    bTa = bTb = shfl64(y2, 2);
+  } else {
+    //  lanes 0, 1, 2: b^-Q, b^-QT, b^-2QT (giant step of the table span Q)
+    ye = 0;
+    if (lane == 0) ye = job->Q;
+    else if (lane == 1) ye = job->Q * T;
+    else if (lane == 2) ye = 2 * job->Q * T;
+    y = vone;
+    if (lane < 3) y = ar.pow_sm(binv, ye);
+    const V gQ_inv = shfl64(y, 0), gT0 = shfl64(y, 1), gT1 = shfl64(y, 2);
+    g0v = gQ_inv;
+    g1v = ar.mul(gQ_inv, gQ_inv);
+    gT0v = gT0;
+    gT1v = gT1;
+    bTa = merged ? bT1 : bT0;
+    bTb = bT1;
+    if (!merged) {
+      // compaction by class: full, even, odd (the reference's sort.cl:40-71)
+      const uint64_t hm0 = ar.mont(h0), hm1 = ar.mont(h1);
+      uint32_t off = 0;
+#pragma unroll
+      for (int c = 1; c <= 3; c++) {
+        const uint64_t m0 = __ballot(cls0 == c), m1 = __ballot(cls1 == c);
+        const uint32_t c0 = __popcll(m0), c1 = __popcll(m1);
+        if (cls0 == c) {
+          KRec e;
+          e.hadj = hm0;
+          e.kidx = lane;
+          e.cls = c;
+          kr[off + lane_rank(m0)] = e;
+        }
+        if (cls1 == c) {
+          KRec e;
+          e.hadj = hm1;
+          e.kidx = lane + 64;
+          e.cls = c;
+          kr[off + c0 + lane_rank(m1)] = e;
+        }
+        nk[c - 1] = c0 + c1;
+        off += c0 + c1;
+      }
+    } else {
+      // One list against a table of even exponent offsets: a target with
+      // shift d searches b^(n - d) = hadj * b^-d.  Full-range k get d = 0 and
+      // d = 1, even-only k d = 0, odd-only k d = 1; cls holds d.
+      uint32_t off = 0;
+#pragma unroll
+      for (int d = 0; d <= 1; d++) {
+        const bool t0 = cls0 == 1 || cls0 == 2 + d, t1 = cls1 == 1 || cls1 == 2 + d;
+        const uint64_t m0 = __ballot(t0), m1 = __ballot(t1);
+        const uint32_t c0 = __popcll(m0), c1 = __popcll(m1);
+        if (t0) {
+          KRec e;
+          e.hadj = ar.mont(d ? ar.mul(h0, binv) : h0);
+          e.kidx = lane;
+          e.cls = d;
+          kr[off + lane_rank(m0)] = e;
+        }
+        if (t1) {
+          KRec e;
+          e.hadj = ar.mont(d ? ar.mul(h1, binv) : h1);
+          e.kidx = lane + 64;
+          e.cls = d;
+          kr[off + c0 + lane_rank(m1)] = e;
+        }
+        off += c0 + c1;
+      }
+      nk[0] = off;
+      if (algo == 2) {  // a small p of a residue-class job: the parity split (S = 2)
+        cS = 2;
+        cE = job->Q;
+        cM = (uint32_t)(((uint64_t)job->nmax - job->nmin + 1 + 2ull * cE - 1) / (2ull * cE));
+        const uint32_t sE = 2 * cE;
+        ye = 0;
+        if (lane == 0) ye = sE;
+        else if (lane == 1) ye = sE * T;
+        y = vone;
+        if (lane < 2) y = ar.pow_sm(binv, ye);
+        g0v = g1v = shfl64(y, 0);
+        gT0v = gT1v = shfl64(y, 1);
+      }
+    }
   }
-  // the stored constants in Montgomery form, converted by lanes 0..8 at once
+  // the stored constants in Montgomery form, converted by lanes 0..7 at once
   V cv = vone;
   switch (lane) {
     case 0: cv = mbase; break;
     case 1: cv = bn; break;
-    case 2: cv = gQ_inv; break;
-    case 3: cv = g1; break;
-    case 4: cv = gT0; break;
-    case 5: cv = gT1; break;
-    case 6: cv = merged ? bT1 : bT0; break;
-    case 7: cv = bT1; break;
+    case 2: cv = g0v; break;
+    case 3: cv = g1v; break;
+    case 4: cv = gT0v; break;
+    case 5: cv = gT1v; break;
+    case 6: cv = bTa; break;
+    case 7: cv = bTb; break;
     default: break;
   }
   const uint64_t cm = ar.mont(cv);
@@ -558,8 +761,12 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   rec.nk[0] = nk[0];
   rec.nk[1] = nk[1];
   rec.nk[2] = nk[2];
-  rec.flags = merged ? 1u : 0u;
-  rec.pad[0] = rec.pad[1] = rec.pad[2] = 0;
+  rec.flags = algo;
+  rec.S = cS;
+  rec.E = cE;
+  rec.M = cM;
+  rec.pad32 = 0;
+  rec.pad = 0;
   if (lane == 0) *R = rec;
 }
 
@@ -570,6 +777,8 @@ __global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t pi = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t ngen = J.nprimes;
+  __shared__ uint64_t s_zv[4][kZV];  // per wavefront: z^i of the residue classes
+  __shared__ uint32_t s_zh[4][kZH];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     atomicAdd((unsigned long long*)&cnt->primecount, (unsigned long long)ngen);
     atomicMax((unsigned long long*)&cnt->max_seg_primes, (unsigned long long)ngen);
@@ -580,12 +789,14 @@ __global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
   const uint64_t p = J.primes[pi];
   const uint64_t q = mont_inv(p);
   const uint64_t one = r_mod(p);
+  uint64_t* zv = s_zv[threadIdx.x >> 6];
+  uint32_t* zh = s_zh[threadIdx.x >> 6];
   if ((p & 1) && p < kFpLimit) {
     FpArith ar;
     ar.p = (double)p;
     ar.pinv = 1.0 / ar.p;
     ar.R = (double)one;
-    setup_prime(ar, J, p, q, one, pi, lane);
+    setup_prime(ar, J, p, q, one, pi, lane, zv, zh);
   } else {
     MontArith ar;
     ar.p = p;
@@ -595,7 +806,7 @@ __global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
 #pragma unroll
     for (int i = 0; i < 5; i++) r2 = mmul(r2, r2, p, q);  // 4^(2^5) = 2^64: Montgomery form of R
     ar.r2 = r2;
-    setup_prime(ar, J, p, q, one, pi, lane);
+    setup_prime(ar, J, p, q, one, pi, lane, zv, zh);
   }
 }
 
@@ -717,6 +928,19 @@ __device__ __forceinline__ uint32_t flt_miss(const uint32_t* flt, uint32_t fmask
   return ~flt[(uint32_t)v & fmask] & flt_bits(v);
 }
 
+// Hash of the match pass's filter over the queued positives.  It must not
+// reuse the main filter's fields: a false positive passed the main filter
+// because some baby value shares its word and bit fields, and that baby value
+// would then pass here too.  A multiplicative mix of all 64 bits, read from
+// its upper bits (which depend on every input bit): word index bits 22..31,
+// bit positions from bits 7..21.
+__device__ __forceinline__ uint32_t qmix(uint64_t v) {
+  return ((uint32_t)v ^ ((uint32_t)(v >> 32) * 0x85EBCA77u)) * 0x9E3779B1u;
+}
+__device__ __forceinline__ uint32_t qmix_bits(uint32_t h) {
+  return (1u << ((h >> 7) & 31u)) | (1u << ((h >> 12) & 31u)) | (1u << ((h >> 17) & 31u));
+}
+
 // Trapezoid self-scheduling of the persistent kernel's work: ticket k (one
 // atomic increment of a shared counter) is the item range [start(k),
 // start(k+1)), chunk sizes falling linearly from f = N/(2B) items (B
@@ -752,7 +976,7 @@ __device__ __forceinline__ uint32_t guide_start(const Guide& g, uint32_t k) {
 // 384 entries keep the static LDS at 13 KB, so a 64 KB filter still fits two
 // 256-thread workgroups per CU.
 constexpr uint32_t kQCap = 384;
-constexpr uint32_t kHQ = 1024;  // power of two, at least twice kQCap
+constexpr uint32_t kHQ = 1024;  // filter words over the queue (qmix gives 10 index bits)
 constexpr uint32_t kHitCap = 128;
 
 // One persistent kernel serves every job of a batch.  An item is (job, prime,
@@ -768,13 +992,16 @@ template <int T>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 : 3))) void k_bsgs(
     SegJob* __restrict__ jobs, BatchState* __restrict__ bs) {
   constexpr int kU = 8;  // targets per giant-step group on the binary64 path at T <= 256
+  constexpr int kLgT = T >= 1024 ? 10 : T >= 512 ? 9 : T >= 256 ? 8 : 7;  // log2(T)
+  static_assert((1 << kLgT) == T, "T is a power of two from 128 to 1024");
+  __shared__ uint64_t s_pw[2][kLgT];  // per-item power tables (see vstart)
   extern __shared__ uint4 lds4[];
   uint32_t* tab = (uint32_t*)lds4;  // the filter; the bucketed table of the overflow recovery reuses it
   __shared__ uint64_t s_h[2 * RS_MAXK + 16];  // targets of the item (+ read-ahead pad)
-  __shared__ uint32_t s_k[2 * RS_MAXK];   // sequence index | shift << 31
+  __shared__ uint32_t s_k[2 * RS_MAXK];   // sequence index | shift << 16
   __shared__ uint32_t s_item, s_job, s_order, s_dups, s_claims, s_qn;
   __shared__ uint4 s_q[kQCap];  // filter positives of the item: residue, target, giant step
-  __shared__ uint32_t s_hq[kHQ];  // open-addressing index over s_q by residue (slot + 1, 0 = empty)
+  __shared__ uint32_t s_hq[kHQ];  // Bloom filter of the queued residues (match pass)
   __shared__ uint2 s_hit[kHitCap];  // baby-step hits of a match pass: (entry, queue slot)
   __shared__ uint32_t s_nhit;
   const uint32_t tid = threadIdx.x;
@@ -836,7 +1063,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     if (item >= nitems) break;
     const SegJob& J = jobs[jj];
     const JobDev* __restrict__ job = J.job;
-    const bool merged = job->algo == 1;
+    const bool merged = job->algo != 0;  // one list per prime (merged or residue-class table)
     const uint32_t local = item - bs->item_base[jj];
     const uint32_t pi = merged ? local : local / 3, c = merged ? 0 : local % 3;
     const PrimeRec* __restrict__ R = J.recs + pi;
@@ -850,11 +1077,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     const uint32_t nmin = job->nmin, nmax = job->nmax;
     const uint32_t cp = (c && !merged) ? 1 : 0;
     const uint64_t p = R->p, q = R->q, one = R->one, mbase = R->mbase;
-    const uint32_t Ql = cp ? job->QQ : job->Q;
-    const uint32_t Ml = cp ? job->MM : job->M;
-    const uint32_t stride = (cp || merged) ? 2 : 1;
+    // table stride, baby-step entries, exponent span and giant steps of the
+    // item: the reference's (three lists), fixed (merged), or per prime
+    // (residue classes)
+    const bool rcls = job->algo == 2;
+    const uint32_t stride = rcls ? R->S : (cp || merged) ? 2 : 1;
+    const uint32_t nent = rcls ? R->E : (cp ? job->QQ : job->Q) / stride;
+    const uint32_t Ql = rcls ? stride * nent : cp ? job->QQ : job->Q;
+    const uint32_t Ml = rcls ? R->M : cp ? job->MM : job->M;
     const uint32_t nstart = nmin + (c == 2 ? 1 : 0);
-    const uint32_t nent = Ql / stride;
     const uint64_t v0 = (c == 2) ? mul_base(R->bn, mbase, base, p, q) : R->bn;
     const uint64_t bstep = R->bT[cp];
     const uint32_t koff = (c >= 1 ? R->nk[0] : 0) + (c >= 2 ? R->nk[1] : 0);
@@ -885,8 +1116,26 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     const bool fp = (p & 1) && p < kFpLimit;
     const double pd = (double)p, pinv = fp ? 1.0 / pd : 0.0;
     const double bd = fp ? (double)mmul(mbase, 1, p, q) : 0.0;  // plain b
-    const uint64_t vstart = fp ? fp_to_u64(fp_mulmod((double)v0, fp_powmod_sm(bd, stride * tid, pd, pinv), pd, pinv))
-                               : mmul(v0, basepow_sm(mbase, stride * tid, base, p, q, one), p, q);
+    // Per-item power tables, one entry per bit of tid: s_pw[0][i] =
+    // b^(stride*2^i) and s_pw[1][i] = G^(2^i) (plain residues as double bits
+    // on the binary64 path, Montgomery residues otherwise), written by
+    // 2*log2(T) threads before the barrier below.  A thread's first baby
+    // value b^(nstart + stride*tid) and giant power G^tid then cost
+    // popcount(tid) products instead of an exponentiation each.
+    const uint64_t Gm = R->g[cp];
+    if (tid < 2 * kLgT) {
+      const uint32_t i = tid % kLgT, side = tid / kLgT;
+      if (fp) {
+        double x = side ? (double)mmul(Gm, 1, p, q) : fp_powmod_sm(bd, stride, pd, pinv);
+        for (uint32_t k = 0; k < i; k++) x = fp_mulmod(x, x, pd, pinv);
+        s_pw[side][i] = (uint64_t)__double_as_longlong(x);
+      } else {
+        uint64_t x = side ? Gm : basepow_sm(mbase, stride, base, p, q, one);
+        for (uint32_t k = 0; k < i; k++) x = mmul(x, x, p, q);
+        s_pw[side][i] = x;
+      }
+    }
+    uint64_t vstart = 0;  // b^(nstart + stride*tid), Montgomery; set after the barrier
     const double bsd = fp ? (double)mmul(bstep, 1, p, q) : 0.0;
     // Every baby value of this thread, e = tid + T*i < emax: fn(e, value),
     // in four interleaved chains (i mod 4) stepping by b^(4*stride*T), so
@@ -917,18 +1166,20 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       }
     };
 
-    // Exact resolution of the queued values s_q[0, nq): an index of them by
-    // residue in LDS, then every baby value e < emax is looked up in it; a
-    // hit is a solution n = nstart + (giant step) * Ql + stride * e + d, or,
-    // with ord != 0 (the order path), n = nstart + stride * e + d + t * ord.
-    // Hits are listed in LDS during the walk and claimed afterwards, so the
-    // walk stays small; a full list sends the whole walk again with claims on
-    // the spot (each hit still claimed once).  Baby values whose low 32 bits
-    // are 0 never match: the reference's table cannot hold them
-    // (giant.cl:44-46, 61-63).
+    // Exact resolution of the queued values s_q[0, nq): a small Bloom filter
+    // of them in LDS (s_hq, kHQ words hashed by qmix), then
+    // every baby value e < emax is tested against it with one LDS read; the
+    // rare value that passes is compared with every queued value.  A hit is
+    // a solution n = nstart + (giant step) * Ql + stride * e + d, or, with
+    // ord != 0 (the order path), n = nstart + stride * e + d + t * ord.  Hits
+    // are listed in LDS during the walk and claimed afterwards, so the walk
+    // stays small; a full list sends the whole walk again with claims on the
+    // spot (each hit still claimed once).  Baby values whose low 32 bits are
+    // 0 never match: the reference's table cannot hold them (giant.cl:44-46,
+    // 61-63).
     auto claim_hit = [&](uint32_t e, uint32_t slot, uint32_t ord) {
       const uint4 qe = s_q[slot];
-      const uint32_t kk = s_k[qe.z], kidx = kk & 0x7FFFFFFFu, d = kk >> 31;
+      const uint32_t kk = s_k[qe.z], kidx = kk & 0xFFFFu, d = kk >> 16;
       if (ord) {
         for (uint64_t n = (uint64_t)nstart + stride * e + d; n <= nmax; n += ord)
           claim(job, bitmap, fac, cnt, p, kidx, (uint32_t)n, &s_dups, &s_claims);
@@ -936,27 +1187,31 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         claim(job, bitmap, fac, cnt, p, kidx, nstart + qe.w * Ql + stride * e + d, &s_dups, &s_claims);
       }
     };
-    auto match = [&](uint32_t nq, uint32_t emax, uint32_t ord) {
+    auto match = [&](uint32_t nq, uint32_t emax, uint32_t ord) __attribute__((always_inline)) {
       for (uint32_t s = tid; s < kHQ; s += T) s_hq[s] = 0u;
       if (tid == 0) s_nhit = 0;
       __syncthreads();
       for (uint32_t s = tid; s < nq; s += T) {
-        uint32_t h = s_q[s].x & (kHQ - 1);
-        while (atomicCAS(&s_hq[h], 0u, s + 1) != 0u) h = (h + 1) & (kHQ - 1);
+        const uint4 qe = s_q[s];
+        const uint32_t h = qmix(((uint64_t)qe.y << 32) | qe.x);
+        atomicOr(&s_hq[h >> 22], qmix_bits(h));
       }
       __syncthreads();
-      baby_walk(emax, [&](uint32_t e, uint64_t v) {
+      // queued values equal to baby value v: slots s with fn(e, s)
+      auto scan = [&](uint32_t e, uint64_t v, auto&& fn) {
         const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-        if (!lo) return;
-        uint32_t h = lo & (kHQ - 1), sl;
-        while ((sl = s_hq[h]) != 0u) {
-          const uint4 qe = s_q[sl - 1];
-          if (qe.x == lo && qe.y == hi) {
-            const uint32_t at = atomicAdd(&s_nhit, 1u);
-            if (at < kHitCap) s_hit[at] = make_uint2(e, sl - 1);
-          }
-          h = (h + 1) & (kHQ - 1);
+        const uint32_t h = qmix(v), m = qmix_bits(h);
+        if (!lo || (s_hq[h >> 22] & m) != m) return;
+        for (uint32_t s = 0; s < nq; s++) {
+          const uint4 qe = s_q[s];
+          if (qe.x == lo && qe.y == hi) fn(e, s);
         }
+      };
+      baby_walk(emax, [&](uint32_t e, uint64_t v) {
+        scan(e, v, [&](uint32_t ee, uint32_t s) {
+          const uint32_t at = atomicAdd(&s_nhit, 1u);
+          if (at < kHitCap) s_hit[at] = make_uint2(ee, s);
+        });
       });
       __syncthreads();
       const uint32_t nh = s_nhit;
@@ -965,10 +1220,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       } else {
         uint64_t v = vstart;  // rare: walk again, claiming on the spot
         for (uint32_t e = tid; e < emax; e += T) {
-          const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-          if (lo)
-            for (uint32_t h = lo & (kHQ - 1), sl; (sl = s_hq[h]) != 0u; h = (h + 1) & (kHQ - 1))
-              if (s_q[sl - 1].x == lo && s_q[sl - 1].y == hi) claim_hit(e, sl - 1, ord);
+          scan(e, v, [&](uint32_t ee, uint32_t s) { claim_hit(ee, s, ord); });
           v = mmul(v, bstep, p, q);
         }
       }
@@ -998,7 +1250,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         const int e = gtab_find_at(gtab, gbmask, mbase, p, q, one, base, nstart, stride, g);
         if (e >= 0) {
           const uint32_t kk = s_k[i];
-          claim(job, bitmap, fac, cnt, p, kk & 0x7FFFFFFFu, nstart + qi * Ql + stride * (uint32_t)e + (kk >> 31),
+          claim(job, bitmap, fac, cnt, p, kk & 0xFFFFu, nstart + qi * Ql + stride * (uint32_t)e + (kk >> 16),
                 &s_dups, &s_claims);
         }
       };
@@ -1040,10 +1292,23 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     }
     for (uint32_t i = tid; i < nk; i += T) {
       s_h[i] = kr[i].hadj;
-      s_k[i] = kr[i].kidx | ((merged ? kr[i].cls : 0u) << 31);
+      s_k[i] = kr[i].kidx | ((merged ? kr[i].cls : 0u) << 16);
     }
     if (tid == 0) s_order = 0xFFFFFFFFu;
     __syncthreads();
+    if (fp) {
+      double v = (double)v0;
+#pragma unroll
+      for (int i = 0; i < kLgT; i++)
+        if ((tid >> i) & 1u) v = fp_mulmod(v, __longlong_as_double(s_pw[0][i]), pd, pinv);
+      vstart = fp_to_u64(v);
+    } else {
+      uint64_t v = v0;
+#pragma unroll
+      for (int i = 0; i < kLgT; i++)
+        if ((tid >> i) & 1u) v = mmul(v, s_pw[0][i], p, q);
+      vstart = v;
+    }
     // baby steps set the filter bits and find ord_p(b) below the span
     {
       const uint64_t v0m = v0;
@@ -1073,7 +1338,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       // tid takes q = tid + r*T in F full rounds, four targets at a time so
       // four products and four filter reads are in flight together.
       const double pc = pd + kFpTwo52;
-      double gq = fp_powmod_sm((double)mmul(R->g[cp], 1, p, q), tid, pd, pinv);
+      double gq = 1.0;  // G^tid, plain
+#pragma unroll
+      for (int i = 0; i < kLgT; i++)
+        if ((tid >> i) & 1u) gq = fp_mulmod(gq, __longlong_as_double(s_pw[1][i]), pd, pinv);
       const double gtd = (double)mmul(R->gT[cp], 1, p, q);
       for (uint32_t i = tid; i < nk; i += T) s_h[i] = (uint64_t)__double_as_longlong((double)s_h[i]);
       __syncthreads();
@@ -1224,7 +1492,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     } else {
       // giant steps with integer Montgomery products, same schedule
       const uint64_t g = R->g[cp], gT = R->gT[cp];
-      uint64_t Gq = powmod_sm(g, tid, p, q, one);
+      uint64_t Gq = one;  // G^tid
+#pragma unroll
+      for (int i = 0; i < kLgT; i++)
+        if ((tid >> i) & 1u) Gq = mmul(Gq, s_pw[1][i], p, q);
       uint32_t r0 = 0;
       bool done = false;  // the overflow recovery finished the item
       for (uint32_t r = 0; r < F; r++) {
@@ -1299,7 +1570,7 @@ __global__ void k_round_items(SegJob* __restrict__ jobs, BatchState* __restrict_
   for (uint32_t j = 0; j < njobs; j++) {
     bs->item_base[j] = base;
     const uint32_t np = jobs[j].nprimes < jobs[j].cap ? jobs[j].nprimes : jobs[j].cap;
-    base += np * (jobs[j].job->algo == 1 ? 1u : 3u);
+    base += np * (jobs[j].job->algo != 0 ? 1u : 3u);
   }
   bs->item_base[njobs] = base;
 }

@@ -187,6 +187,22 @@ def load_model(ctx, path=MODEL_PATH):
     return m
 
 
+def model_pick(ctx, cands, base, kcount, nterms, span, log2p, path=MODEL_PATH):
+    """The committed cost model's choice among the candidate list, or None
+    when there is no model or it was trained on another candidate list."""
+    try:
+        model = load_model(ctx, path)
+    except (OSError, KeyError):
+        return None
+    if model.n_out != len(cands):
+        model.mlp.close()
+        return None
+    x = features(base, kcount, nterms, span, log2p)[None, :]
+    idx = int(model.choose(x, np.array([[c.threads != 0 for c in cands]]))[0])
+    model.mlp.close()
+    return idx
+
+
 def best_fixed(Y, V):
     """The non-learned rule fitted to the same records: the one candidate with
     the lowest mean log time over the rows where it ran."""

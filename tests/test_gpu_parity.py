@@ -175,19 +175,21 @@ def test_positive_queue_overflow_is_recovered(ctx, case, qcap):
     gpu = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, prm)
     orc = oracle_run(base, ks, cs, nmin, nmax, bm, pmin, pmax)
     _compare(base, ks, cs, nmin, nmax, wps, bm, gpu, orc)
-    mprm = merged_params(ctx, nmin, nmax, K, 2048, 4096, 512)
-    mprm.queue_cap = qcap
-    gcnt, gfac, gbm = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, mprm)
-    assert gcnt["claims"] == orc[0]["claims"]
-    assert np.array_equal(gbm, orc[2])
+    for algo in (R.ALGO_MERGED, R.ALGO_CLASSES):
+        mprm = merged_params(ctx, nmin, nmax, K, 2048, 4096, 512, algo=algo)
+        mprm.queue_cap = qcap
+        gcnt, gfac, gbm = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, mprm)
+        assert gcnt["claims"] == orc[0]["claims"]
+        assert np.array_equal(gbm, orc[2])
 
 
-def merged_params(ctx, nmin, nmax, K, entries=4096, hsize=8192, threads=256):
-    """Parameters of the merged-table path: one table of 'entries' even
-    exponent offsets (span 2*entries)."""
+def merged_params(ctx, nmin, nmax, K, entries=4096, hsize=8192, threads=256, algo=R.ALGO_MERGED):
+    """Parameters of the one-list paths: the merged table of 'entries' even
+    exponent offsets (span 2*entries), or residue-class tables of at most
+    'entries' entries."""
     prm = R.reference_params(nmin, nmax, K, 65536, ctx.compute_units)
-    prm.algo = R.ALGO_MERGED
-    prm.Q = 2 * entries
+    prm.algo = algo
+    prm.Q = 2 * entries if algo == R.ALGO_MERGED else entries
     prm.hsize = hsize
     prm.threads = threads
     return prm
@@ -196,15 +198,18 @@ def merged_params(ctx, nmin, nmax, K, entries=4096, hsize=8192, threads=256):
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("shape", [(1024, 2048, 128), (4096, 8192, 256), (8192, 16384, 512)],
                          ids=["e1024", "e4096", "e8192"])
-def test_merged_table_same_eliminations(ctx, case, shape):
-    """The merged-table BSGS eliminates exactly the terms the reference-shaped
-    sieve (and the oracle) eliminates; its duplicate count follows its own
-    search windows, so it is not compared."""
+@pytest.mark.parametrize("algo", [R.ALGO_MERGED, R.ALGO_CLASSES], ids=["merged", "classes"])
+def test_merged_table_same_eliminations(ctx, case, shape, algo):
+    """The one-list BSGS shapes eliminate exactly the terms the
+    reference-shaped sieve (and the oracle) eliminates: the merged table, and
+    the residue-class tables, whose stride and length change from prime to
+    prime (p below 1024 keep the parity split).  Their duplicate counts
+    follow their own search windows, so they are not compared."""
     name, base, K, nmin, nmax, pmin, pmax, seed, dens = case
-    print("merged case", case, shape, flush=True)
+    print("one-list case", case, shape, algo, flush=True)
     ks, cs = _draw(base, K, seed)
     bm, wps = full_bitmap(K, nmin, nmax, density=dens, seed=seed)
-    prm = merged_params(ctx, nmin, nmax, K, *shape)
+    prm = merged_params(ctx, nmin, nmax, K, *shape, algo=algo)
     gcnt, gfac, gbm = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, prm)
     ocnt, ofac, obm = oracle_run(base, ks, cs, nmin, nmax, bm, pmin, pmax)
     assert gcnt["primecount"] == ocnt["primecount"]
@@ -229,6 +234,8 @@ def test_batch_of_uneven_jobs_matches_oracle(ctx):
         (6, 3, 500, 2500, 300, 70000, 24, R.ALGO_MERGED, 1024, 4096),
         (10, 25, 40000, 60000, 10**6, 10**6 + 4 * 10**5, 25, R.ALGO_MERGED, 4096, 16384),
         (2, 100, 3000000, 3010000, 2**48, 2**48 + 2 * 10**5, 26, R.ALGO_THREE_LISTS, 0, 0),
+        (6, 40, 20000, 90000, 2**40, 2**40 + 2 * 10**5, 27, R.ALGO_CLASSES, 4096, 8192),
+        (7, 9, 100, 60000, 2**62, 2**62 + 10**5, 28, R.ALGO_CLASSES, 2048, 2048),
     ]
     jobs, inputs = [], []
     for (base, K, nmin, nmax, pmin, pmax, seed, algo, ent, hs) in specs:
@@ -237,6 +244,8 @@ def test_batch_of_uneven_jobs_matches_oracle(ctx):
         prm = R.reference_params(nmin, nmax, K, 65536, ctx.compute_units)
         if algo == R.ALGO_MERGED:
             prm.algo, prm.Q, prm.hsize = algo, 2 * ent, hs
+        elif algo == R.ALGO_CLASSES:
+            prm.algo, prm.Q, prm.hsize = algo, ent, hs
         jobs.append(ctx.job(base, ks, cs, nmin, nmax, bm, wps, prm))
         inputs.append((base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, algo))
     print("batch of", len(jobs), flush=True)
@@ -283,7 +292,8 @@ def test_high_p_production_shape_batched_and_tuned(ctx):
     Montgomery path near the top of the 64-bit range): the reference-shaped
     sieve equals the oracle counter for counter, and the batched path (two
     views of the job in one batch) with the cost model's configuration (the
-    largest merged table when no model is committed) leaves the same bitmap."""
+    largest residue-class table without a model for this candidate list)
+    leaves the same bitmap."""
     base = 5
     ks, cs, nmin, nmax, bm, wps = _high_p_data()
     K = len(ks)
@@ -301,14 +311,11 @@ def test_high_p_production_shape_batched_and_tuned(ctx):
     assert gcnt["claims"] == ocnt["claims"] and gcnt["dups"] == ocnt["dups"]
     assert np.array_equal(gbm, obm)
 
+    from rssieve_amd import tuning
     cands = ctx.tune_candidates(nmin, nmax, K)
-    try:
-        from rssieve_amd import tuning
-        model = tuning.load_model(ctx)
-        x = tuning.features(base, K, int(np.bitwise_count(bm).sum()), nmax - nmin + 1, 63.0)[None, :]
-        idx = int(model.choose(x, np.array([[c.threads != 0 for c in cands]]))[0])
-    except (OSError, KeyError):
-        idx = max((i for i, c in enumerate(cands) if c.algo == R.ALGO_MERGED and c.threads), key=lambda i: cands[i].Q)
+    idx = tuning.model_pick(ctx, cands, base, K, int(np.bitwise_count(bm).sum()), nmax - nmin + 1, 63.0)
+    if idx is None:
+        idx = max((i for i, c in enumerate(cands) if c.algo == R.ALGO_CLASSES and c.threads), key=lambda i: cands[i].Q)
     print("tuned candidate", idx, cands[idx].as_dict(), flush=True)
     job = ctx.job(base, ks, cs, nmin, nmax, bm, wps, cands[idx])
     view = job.view()

@@ -1,0 +1,131 @@
+"""The residue-class rule of the setup's class tables (RS_ALGO_CLASSES),
+checked by brute force.
+
+For L = gcd(p - 1, 240) and z = b^((p-1)/L), every n with b^n = h (mod p)
+satisfies z^n = y with y = h^((p-1)/L), so n = x (mod s) for the x < s = ord(z)
+with z^x = y, and there is no n when y is not a power of z.  The setup keeps
+the reference's own tests (setup.cl:59-87: the parity classes and the
+power-residue tests) and only narrows the n searched to that class, so the
+solutions found in [nmin, nmax] must be exactly those of the reference's
+class.  Composite base-2 strong pseudoprimes are included: there the
+reference's Euler-criterion tests are not sound, and the rule must still
+leave its result unchanged.  Restated here in Python as the kernel states it
+(rs_kernels.hip, setup_prime); the GPU parity tests check the kernel itself.
+"""
+import math
+import random
+
+import pytest
+
+PRD = [3, 4, 5, 7, 8, 9, 11, 13, 17, 19, 23, 29]  # setup.cl:22-48
+
+
+def _sprp2(n):
+    d, t = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        t += 1
+    x = pow(2, d, n)
+    if x in (1, n - 1):
+        return True
+    for _ in range(t - 1):
+        x = x * x % n
+        if x == n - 1:
+            return True
+    return False
+
+
+def _sign(a, p):
+    return 1 if a == 1 else (-1 if a == p - 1 else 0)
+
+
+def _reference_classes(p, b, hs):
+    """cls per target: 0 no search, 1 all n, 2 even n, 3 odd n (setup.cl:59-87, 180-208)."""
+    pm = p - 1
+    resg = _sign(pow(b, pm // 2, p), p)
+    dpass = [d for d in PRD if pm % d == 0 and pow(b, pm // d, p) == 1]
+    out = []
+    for h in hs:
+        cls = 1
+        for d in dpass:
+            if _sign(pow(h, pm // d, p), p) != 1:
+                cls = 0
+        t = _sign(pow(h, pm // 2, p), p)
+        if cls:
+            if resg == -1:
+                cls = 2 if t == 1 else (3 if t == -1 else 0)
+            elif resg == 1 and t == -1:
+                cls = 0
+        out.append(cls)
+    return out
+
+
+def _class_residues(p, b, hs, clss):
+    """(S, residues n mod S per target) as the setup computes them."""
+    pm = p - 1
+    v2 = (pm & -pm).bit_length() - 1
+    L = (1 << min(v2, 4)) * (3 if pm % 3 == 0 else 1) * (5 if pm % 5 == 0 else 1)
+    z = pow(b, pm // L, p)
+    s = next((i for i in range(1, L + 1) if pow(z, i, p) == 1), 0)
+    any_par = any(c >= 2 for c in clss)
+    S = 2 if s == 0 else (2 * s if (s & 1) and any_par else s)
+    table = {pow(z, i, p): i for i in range(s)}
+    res = []
+    for h, cls in zip(hs, clss):
+        rr = []
+        if cls and s == 0:
+            rr = [0, 1] if cls == 1 else [cls - 2]
+        elif cls:
+            x = table.get(pow(h, pm // L, p))
+            if x is not None:
+                if S == s:
+                    if not (cls >= 2 and (x & 1) != cls - 2):
+                        rr = [x]
+                elif cls == 1:
+                    rr = [x, x + s]
+                else:
+                    rr = [x if (x & 1) == cls - 2 else x + s]
+        res.append(rr)
+    return S, res
+
+
+def _cases(limit, nprimes, seed):
+    rng = random.Random(seed)
+    cands = [p for p in range(1025, limit, 2) if _sprp2(p)]
+    comp = [p for p in cands if any(p % q == 0 for q in range(3, math.isqrt(p) + 1, 2))]
+    prime = [p for p in cands if p not in comp]
+    return comp, rng.sample(prime, nprimes)
+
+
+def test_pseudoprimes_are_present():
+    comp, _ = _cases(60000, 1, 1)
+    assert comp[:4] == [2047, 3277, 4033, 4681]
+
+
+@pytest.mark.parametrize("base", [2, 3, 5, 6, 7, 10])
+def test_residue_classes_keep_the_reference_solutions(base):
+    comp, prime = _cases(60000, 40, base)
+    rng = random.Random(base)
+    narrowed = checked = 0
+    for p in comp + prime:
+        if math.gcd(base, p) != 1:
+            continue
+        ks = [k for k in (rng.randrange(2, 10**6) for _ in range(8)) if math.gcd(k, p) == 1]
+        cs = [rng.choice((1, -1)) for _ in ks]
+        hs = [(-c * pow(k, -1, p)) % p for k, c in zip(ks, cs)]
+        clss = _reference_classes(p, base, hs)
+        S, res = _class_residues(p, base, hs, clss)
+        nmin, nmax = 1000, 1000 + p
+        powb = [pow(base, nmin, p)]
+        for _ in range(nmax - nmin):
+            powb.append(powb[-1] * base % p)
+        for h, cls, rr in zip(hs, clss, res):
+            if not cls:
+                continue
+            sols = [nmin + i for i, v in enumerate(powb) if v == h]
+            ref = {n for n in sols if cls == 1 or n % 2 == cls - 2}
+            ours = {n for n in sols if n % S in rr}
+            assert ours == ref, (p, base, h, cls, S, rr)
+            checked += 1
+            narrowed += len(rr) * 2 < S * (2 if cls == 1 else 1)
+    assert checked > 100 and narrowed > checked // 4
