@@ -34,7 +34,8 @@ struct PrimeRec {
   uint64_t bn;                // b^NMIN (Montgomery)
   uint64_t g[2];              // giant step: b^-Q (full) and b^-QQ (parity)
   uint64_t gT[2];             // g[c]^threads
-  uint64_t bT[2];             // b^threads and b^(2*threads): baby stride per round
+  uint64_t bT[2];             // baby stride per round b^(stride*threads) (bT[0]; three lists: b^threads,
+                              // b^(2*threads)); residue classes: bT[1] = b^S
   uint32_t nk[RS_NCLASS];     // k count per class: full, even, odd (0 = skipped)
   uint32_t flags;             // the job's algo when written
   // residue-class tables (algo 2): baby values b^(NMIN + S*e), e < E; M giant

@@ -636,7 +636,8 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     cS = S;
     cE = (uint32_t)E;
     cM = (uint32_t)M;
-    // giant step b^-(S*E), its T-th power, and the baby stride b^(S*T)
+    // giant step b^-(S*E), its T-th power, the baby stride b^(S*T) and the
+    // table stride b^S (kept in bT[1])
     uint64_t ye2 = 0;
     V yb2 = binv;
     if (lane == 0) ye2 = SE;
@@ -644,12 +645,16 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     else if (lane == 2) {
       ye2 = (uint64_t)S * T;
       yb2 = mbase;
+    } else if (lane == 3) {
+      ye2 = S;
+      yb2 = mbase;
     }
     V y2 = vone;
-    if (lane < 3) y2 = ar.pow_u64(yb2, ye2);
+    if (lane < 4) y2 = ar.pow_u64(yb2, ye2);
     g0v = g1v = shfl64(y2, 0);
     gT0v = gT1v = shfl64(y2, 1);
-    bTa = bTb = shfl64(y2, 2);
+    bTa = shfl64(y2, 2);
+    bTb = shfl64(y2, 3);
   } else {
     //  lanes 0, 1, 2: b^-Q, b^-QT, b^-2QT (giant step of the table span Q)
     ye = 0;
@@ -729,6 +734,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
         if (lane < 2) y = ar.pow_sm(binv, ye);
         g0v = g1v = shfl64(y, 0);
         gT0v = gT1v = shfl64(y, 1);
+        bTb = ar.mul(mbase, mbase);  // the table stride b^S, S = 2
       }
     }
   }
@@ -1126,11 +1132,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     if (tid < 2 * kLgT) {
       const uint32_t i = tid % kLgT, side = tid / kLgT;
       if (fp) {
-        double x = side ? (double)mmul(Gm, 1, p, q) : fp_powmod_sm(bd, stride, pd, pinv);
+        double x = side ? (double)mmul(Gm, 1, p, q)
+                        : rcls ? (double)mmul(R->bT[1], 1, p, q) : fp_powmod_sm(bd, stride, pd, pinv);
         for (uint32_t k = 0; k < i; k++) x = fp_mulmod(x, x, pd, pinv);
         s_pw[side][i] = (uint64_t)__double_as_longlong(x);
       } else {
-        uint64_t x = side ? Gm : basepow_sm(mbase, stride, base, p, q, one);
+        uint64_t x = side ? Gm : rcls ? R->bT[1] : basepow_sm(mbase, stride, base, p, q, one);
         for (uint32_t k = 0; k < i; k++) x = mmul(x, x, p, q);
         s_pw[side][i] = x;
       }
