@@ -461,26 +461,34 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     *t0 = (a0 == vone) - (a0 == nmo);
     *t1 = (a1 == vone) - (a1 == nmo);
   };
+  // With the residue classes, y = h^((p-1)/L) is needed anyway, and for d | L
+  // the character h^((p-1)/d) is y^(L/d) (the same value from a short
+  // exponent): the reference's quadratic test and its power-residue tests
+  // with d | L take it from y.
+  V y0 = vone, y1 = vone;
+  if (split) ar.pow_lg2(h0, h1, exL, &y0, &y1);
+  auto sign_from_y = [&](uint32_t e, int* t0, int* t1) {
+    const V a0 = ar.pow_u64(y0, e), a1 = ar.pow_u64(y1, e);
+    *t0 = (a0 == vone) - (a0 == nmo);
+    *t1 = (a1 == vone) - (a1 == nmo);
+  };
   int cls0 = act0 ? 1 : 0, cls1 = act1 ? 1 : 0;
   uint64_t dm = dmask;
   while (dm) {
     const int i = __ffsll((long long)dm) - 1;
     dm &= dm - 1;
     int t0, t1;
-    pow_sign2(shfl64(ex, i + 1), &t0, &t1);
+    const uint32_t d = g_prd[i];
+    if (split && L % d == 0) sign_from_y(L / d, &t0, &t1);
+    else pow_sign2(shfl64(ex, i + 1), &t0, &t1);
     if (t0 != 1) cls0 = 0;
     if (t1 != 1) cls1 = 0;
   }
-  // the quadratic character h^((p-1)/2); with the residue classes it is
-  // y^(L/2) for the y = h^((p-1)/L) the classes need anyway (same value)
-  V y0 = vone, y1 = vone;
+  // the quadratic character h^((p-1)/2)
   {
     int t0, t1;
     if (split) {
-      ar.pow_lg2(h0, h1, exL, &y0, &y1);
-      const V a0 = ar.pow_u64(y0, L / 2), a1 = ar.pow_u64(y1, L / 2);
-      t0 = (a0 == vone) - (a0 == nmo);
-      t1 = (a1 == vone) - (a1 == nmo);
+      sign_from_y(L / 2, &t0, &t1);
     } else {
       pow_sign2(pm >> 1, &t0, &t1);
     }
@@ -1006,6 +1014,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
   __shared__ uint64_t s_h[2 * RS_MAXK + 16];  // targets of the item (+ read-ahead pad)
   __shared__ uint32_t s_k[2 * RS_MAXK];   // sequence index | shift << 16
   __shared__ uint32_t s_item, s_job, s_order, s_dups, s_claims, s_qn;
+  __shared__ uint32_t s_next;  // the block's next item when it is known (same chunk and job), else ~0
   __shared__ uint4 s_q[kQCap];  // filter positives of the item: residue, target, giant step
   __shared__ uint32_t s_hq[kHQ];  // Bloom filter of the queued residues (match pass)
   __shared__ uint2 s_hit[kHitCap];  // baby-step hits of a match pass: (entry, queue slot)
@@ -1059,6 +1068,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         while (it >= jend) jend = bs->item_base[++jcur + 1];
       s_item = it;
       s_job = jcur;
+      s_next = (ci < ce && ci < jend) ? ci : 0xFFFFFFFFu;
       s_dups = 0;
       s_claims = 0;
       s_qn = 0;
@@ -1316,6 +1326,23 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         if ((tid >> i) & 1u) v = mmul(v, s_pw[0][i], p, q);
       vstart = v;
     }
+    // Touch the next item's record and target lines (one 128-byte line per
+    // thread) so that they are in this XCD's L2 when it starts; the value is
+    // only consumed by an empty asm at the end of the item, so nothing waits
+    // for these loads on the way.
+    uint32_t pf = 0;
+    {
+      const uint32_t nx = s_next;
+      const uint32_t kbytes = (merged ? 2u * K : (uint32_t)K) * (uint32_t)sizeof(KRec);
+      if (nx != 0xFFFFFFFFu && (tid == 0 || 128u * (tid - 1) < kbytes)) {
+        const uint32_t lx = nx - bs->item_base[jj];
+        const uint32_t px = merged ? lx : lx / 3;
+        const uint32_t* a = tid == 0 ? (const uint32_t*)(J.recs + px)
+                                     : (const uint32_t*)((const char*)(J.krecs + (size_t)px * (merged ? 2 * K : K)) +
+                                                         128u * (tid - 1));
+        pf = *a;
+      }
+    }
     // baby steps set the filter bits and find ord_p(b) below the span
     {
       const uint64_t v0m = v0;
@@ -1552,6 +1579,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       work = (unsigned long long)nent + (unsigned long long)Ml * (nk + 1);
       probes = (unsigned long long)Ml * nk;
     }
+    asm volatile("" ::"v"(pf));  // the prefetch's only use
     __syncthreads();
     if (tid == 0) {
       if (cnt != s_acc_cnt) {
