@@ -592,26 +592,32 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     uint32_t ra[2] = {0, 0}, rb[2] = {0, 0};
     const int na = classes(cls0, y0, ra), nb = classes(cls1, y1, rb);
     const uint32_t nmod = job->nmin % S;
+    // b^-d for the shifts d < S <= 480: lane j holds b^-j (j < 64) and
+    // b^-(64i) for i = j mod 8, so b^-d is two shuffles and one product
+    // (taken with every lane active, outside the divergent stores)
+    const V bl = ar.pow_u64(binv, lane);
+    const V bh = ar.pow_u64(ar.mul(shfl64(bl, 63), binv), lane & 7u);
+    auto binv_pow = [&](uint32_t d) { return ar.mul(shfl64(bl, (int)(d & 63u)), shfl64(bh, (int)(d >> 6))); };
     uint32_t off = 0;
 #pragma unroll
     for (int a = 0; a < 2; a++) {
       const bool va = na > a, vb = nb > a;
+      const uint32_t da = (ra[a] + S - nmod) % S, db = (rb[a] + S - nmod) % S;
+      const V ta = ar.mul(h0, binv_pow(da)), tb = ar.mul(h1, binv_pow(db));
       const uint64_t m0 = __ballot(va), m1 = __ballot(vb);
       const uint32_t c0 = __popcll(m0), c1 = __popcll(m1);
       if (va) {
-        const uint32_t d = (ra[a] + S - nmod) % S;
         KRec e;
-        e.hadj = ar.mont(d ? ar.mul(h0, ar.pow_u64(binv, d)) : h0);
+        e.hadj = ar.mont(ta);
         e.kidx = lane;
-        e.cls = d;
+        e.cls = da;
         kr[off + lane_rank(m0)] = e;
       }
       if (vb) {
-        const uint32_t d = (rb[a] + S - nmod) % S;
         KRec e;
-        e.hadj = ar.mont(d ? ar.mul(h1, ar.pow_u64(binv, d)) : h1);
+        e.hadj = ar.mont(tb);
         e.kidx = lane + 64;
-        e.cls = d;
+        e.cls = db;
         kr[off + c0 + lane_rank(m1)] = e;
       }
       off += c0 + c1;
@@ -1326,14 +1332,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         if ((tid >> i) & 1u) v = mmul(v, s_pw[0][i], p, q);
       vstart = v;
     }
-    // Touch the next item's record and target lines (one 128-byte line per
-    // thread) so that they are in this XCD's L2 when it starts; the value is
-    // only consumed by an empty asm at the end of the item, so nothing waits
-    // for these loads on the way.
+    // Touch the next item's record and its first K target records' lines
+    // (one 128-byte line per thread) so that they are in this XCD's L2 when
+    // it starts; the value is only consumed by an empty asm at the end of the
+    // item, so nothing waits for these loads on the way.
     uint32_t pf = 0;
     {
       const uint32_t nx = s_next;
-      const uint32_t kbytes = (merged ? 2u * K : (uint32_t)K) * (uint32_t)sizeof(KRec);
+      // the targets of a prime rarely exceed K (room is 2K): K records' lines
+      const uint32_t kbytes = (uint32_t)K * (uint32_t)sizeof(KRec);
       if (nx != 0xFFFFFFFFu && (tid == 0 || 128u * (tid - 1) < kbytes)) {
         const uint32_t lx = nx - bs->item_base[jj];
         const uint32_t px = merged ? lx : lx / 3;
