@@ -269,19 +269,23 @@ __device__ __forceinline__ void fp_powmod_lg2(double x0, double x1, uint64_t exp
 // d of the power-residue tests (setup.cl:22-48)
 __constant__ uint32_t g_prd[12] = {3, 4, 5, 7, 8, 9, 11, 13, 17, 19, 23, 29};
 
-// Residue classes of n (algo 2).  With L = gcd(p - 1, 240) and z =
+// Residue classes of n (algo 2).  With L = gcd(p - 1, 5040) and z =
 // b^((p-1)/L), every solution of b^n = h (mod p) satisfies z^n = y with y =
-// h^((p-1)/L).  The powers z^i, i < s = ord(z), are distinct, so y = z^x
-// fixes n = x (mod s), and a y that is no power of z leaves no n at all.
-// This is algebra mod p, true for a composite 2-PRP as well: the reference's
-// tests (parity classes, power residues, setup.cl:59-87) still decide which
-// k are searched, and the classes only narrow the n they are searched on, so
-// the solutions found are the same.  For a prime p with b a quadratic
+// h^((p-1)/L).  Split L = La * Lb with La = gcd(p - 1, 80) and Lb =
+// gcd(p - 1, 63): for za = z^Lb of order sa (the first i > 0 with za^i = 1)
+// the powers za^i, i < sa, are distinct, so ya = y^Lb = za^xa fixes
+// n = xa (mod sa), and likewise n = xb (mod sb) from zb = z^La, yb = y^La;
+// when sa and sb are coprime (always for a prime p) the two combine by CRT
+// into n mod s, s = sa*sb = ord(z), and a y that is no power of z leaves no n
+// at all.  This is algebra mod p, true for a composite 2-PRP as well (whose
+// non-coprime orders fall back to the parity split): the reference's tests
+// (parity classes, power residues, setup.cl:59-87) still decide which k are
+// searched, and the classes only narrow the n they are searched on, so the
+// solutions found are the same.  For a prime p with b a quadratic
 // non-residue s is even and fixes the parity the reference's class gives.
-constexpr uint32_t kZL = 240;                 // L divides 240 = 16 * 3 * 5
-constexpr uint32_t kZV = 256;                 // z^i table entries (>= kZL + 1)
-constexpr uint32_t kZH = 512;                 // hash slots over them (power of two)
-constexpr uint64_t kClassMinP = 1024;         // smaller p keep the merged table's parity split
+constexpr uint32_t kZV = 256;                 // za^i at [0, 81), zb^i at [128, 192)
+constexpr uint32_t kZH = 512;                 // their hash slots: [0, 256) and [256, 384)
+constexpr uint64_t kClassMinP = 16384;        // smaller p keep the merged table's parity split (exponent (p-1)/L >= 2)
 
 // LDS operations of one wavefront become visible to its other lanes
 __device__ __forceinline__ void wave_sync() {
@@ -395,14 +399,16 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   const V AB = ar.mul(A0, B0);
   const V total = ar.mul(AB, gQ);  // prefix[KCOUNT] of setup.cl:147-151
 
-  // residue classes: L = gcd(p - 1, 240)
+  // residue classes: L = La * Lb = gcd(p - 1, 80) * gcd(p - 1, 63)
   const uint64_t pm = p - 1;
   const bool split = algo == 2 && p > kClassMinP;
-  uint32_t L = 2;
+  uint32_t L = 2, La = 2, Lb = 1;
   uint64_t exL = 0;
   if (split) {
     const uint32_t v2 = (uint32_t)__builtin_ctzll(pm);
-    L = (1u << (v2 < 4 ? v2 : 4)) * (pm % 3 == 0 ? 3u : 1u) * (pm % 5 == 0 ? 5u : 1u);
+    La = (1u << (v2 < 4 ? v2 : 4)) * (pm % 5 == 0 ? 5u : 1u);
+    Lb = (pm % 9 == 0 ? 9u : pm % 3 == 0 ? 3u : 1u) * (pm % 7 == 0 ? 7u : 1u);
+    L = La * Lb;
     exL = pm / L;
   }
 
@@ -465,13 +471,20 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   // the character h^((p-1)/d) is y^(L/d) (the same value from a short
   // exponent): the reference's quadratic test and its power-residue tests
   // with d | L take it from y.
-  V y0 = vone, y1 = vone;
-  if (split) ar.pow_lg2(h0, h1, exL, &y0, &y1);
-  auto sign_from_y = [&](uint32_t e, int* t0, int* t1) {
-    const V a0 = ar.pow_u64(y0, e), a1 = ar.pow_u64(y1, e);
+  V y0 = vone, y1 = vone, ya0 = vone, ya1 = vone, yb0 = vone, yb1 = vone;
+  if (split) {
+    ar.pow_lg2(h0, h1, exL, &y0, &y1);
+    ya0 = ar.pow_u64(y0, Lb);
+    ya1 = ar.pow_u64(y1, Lb);
+    yb0 = ar.pow_u64(y0, La);
+    yb1 = ar.pow_u64(y1, La);
+  }
+  auto sign_of = [&](V x0, V x1, uint32_t e, int* t0, int* t1) {
+    const V a0 = ar.pow_u64(x0, e), a1 = ar.pow_u64(x1, e);
     *t0 = (a0 == vone) - (a0 == nmo);
     *t1 = (a1 == vone) - (a1 == nmo);
   };
+  auto sign_from_y = [&](uint32_t e, int* t0, int* t1) { sign_of(y0, y1, e, t0, t1); };
   int cls0 = act0 ? 1 : 0, cls1 = act1 ? 1 : 0;
   uint64_t dm = dmask;
   while (dm) {
@@ -488,7 +501,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   {
     int t0, t1;
     if (split) {
-      sign_from_y(L / 2, &t0, &t1);
+      sign_of(ya0, ya1, La / 2, &t0, &t1);  // y^(L/2) = (y^Lb)^(La/2), Lb odd
     } else {
       pow_sign2(pm >> 1, &t0, &t1);
     }
@@ -528,41 +541,78 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   V g0v, g1v, gT0v, gT1v, bTa, bTb;
   KRec* kr = J.krecs + (size_t)pi * (merged ? 2 * K : K);
   if (split) {
-    // the table of z^i, 0 <= i <= L, and s = ord(z) (0: z^i != 1 for 0 < i <= L,
-    // possible only for a composite p; then the classes are the parity split)
+    // the tables of za^i (i <= La, at zv[0..]) and zb^i (i <= Lb, at
+    // zv[128..]), and their orders sa, sb (0: no i > 0 with value 1 in range,
+    // possible only for a composite p)
     for (uint32_t h = lane; h < kZH; h += 64) zh[h] = 0u;
-    V cur = ar.pow_u64(zr, lane);
-    const V z64 = ar.mul(shfl64(cur, 63), zr);
-    uint32_t s = 0;
-    for (uint32_t j = 0; 64 * j <= L; j++) {
-      const uint32_t idx = 64 * j + lane;
-      const bool in = idx <= L;
-      if (in) zv[idx] = ar.bits(cur);
-      const uint64_t m1 = __ballot(in && idx > 0 && cur == vone);
-      if (!s && m1) s = 64 * j + (uint32_t)__ffsll((long long)m1) - 1;
-      cur = ar.mul(cur, z64);
-    }
-    wave_sync();
-    for (uint32_t idx = lane; idx < s; idx += 64) {
-      uint32_t h = (uint32_t)zv[idx] & (kZH - 1);
-      while (atomicCAS(&zh[h], 0u, idx + 1) != 0u) h = (h + 1) & (kZH - 1);
-    }
-    wave_sync();
-    // discrete log of y to the base z (index in the table), or -1
-    auto dlog = [&](V yv) -> int {
-      const uint64_t key = ar.bits(yv);
-      for (uint32_t h = (uint32_t)key & (kZH - 1);; h = (h + 1) & (kZH - 1)) {
-        const uint32_t sl = zh[h];
-        if (!sl) return -1;
-        if (zv[sl - 1] == key) return (int)sl - 1;
+    const V za = ar.pow_u64(zr, Lb), zb = ar.pow_u64(zr, La);
+    uint32_t sa = 0, sb = 0;
+    {
+      V cur = ar.pow_u64(za, lane);
+      const V z64 = ar.mul(shfl64(cur, 63), za);
+      for (uint32_t j = 0; 64 * j <= La; j++) {
+        const uint32_t idx = 64 * j + lane;
+        const bool in = idx <= La;
+        if (in) zv[idx] = ar.bits(cur);
+        const uint64_t m1 = __ballot(in && idx > 0 && cur == vone);
+        if (!sa && m1) sa = 64 * j + (uint32_t)__ffsll((long long)m1) - 1;
+        cur = ar.mul(cur, z64);
       }
+      const V cb = ar.pow_u64(zb, lane);  // lane <= 63 covers i <= Lb
+      if (lane <= Lb) zv[128 + lane] = ar.bits(cb);
+      const uint64_t m1 = __ballot(lane <= Lb && lane > 0 && cb == vone);
+      if (m1) sb = (uint32_t)__ffsll((long long)m1) - 1;
+    }
+    uint32_t g = sa, h2 = sb;  // gcd(sa, sb)
+    while (h2) {
+      const uint32_t t = g % h2;
+      g = h2;
+      h2 = t;
+    }
+    const uint32_t s = (sa && sb && g == 1) ? sa * sb : 0u;
+    wave_sync();
+    if (s) {
+      for (uint32_t idx = lane; idx < sa; idx += 64) {
+        uint32_t h = (uint32_t)zv[idx] & 255u;
+        while (atomicCAS(&zh[h], 0u, idx + 1) != 0u) h = (h + 1) & 255u;
+      }
+      if (lane < sb) {
+        uint32_t h = (uint32_t)zv[128 + lane] & 127u;
+        while (atomicCAS(&zh[256 + h], 0u, lane + 1) != 0u) h = (h + 1) & 127u;
+      }
+    }
+    wave_sync();
+    // sa^-1 mod sb for the CRT (sb <= 63)
+    uint32_t sainv = 0;
+    if (s)
+      for (uint32_t t = 0; t < sb; t++)
+        if ((sa * t) % sb == 1u % sb) {
+          sainv = t;
+          break;
+        }
+    // index of a key in one table (hash base hb, mask hm, values at vb), or -1
+    auto lookup = [&](uint64_t key, uint32_t hb, uint32_t hm, uint32_t vb) -> int {
+      for (uint32_t h = (uint32_t)key & hm;; h = (h + 1) & hm) {
+        const uint32_t sl = zh[hb + h];
+        if (!sl) return -1;
+        if (zv[vb + sl - 1] == key) return (int)sl - 1;
+      }
+    };
+    // the x < s with z^x = y, from ya = y^Lb and yb = y^La, or -1
+    auto dlog = [&](V yav, V ybv) -> int {
+      const int xa = lookup(ar.bits(yav), 0u, 255u, 0u);
+      if (xa < 0) return -1;
+      const int xb = lookup(ar.bits(ybv), 256u, 127u, 128u);
+      if (xb < 0) return -1;
+      const uint32_t dx = ((uint32_t)xb + sb - (uint32_t)xa % sb) % sb;
+      return xa + (int)(sa * ((dx * sainv) % sb));
     };
     // Table stride S: s when s is even (it fixes the parity) or no k has a
     // parity class here, else 2s (CRT of the class mod s and the parity);
     // 2 without a usable s.  Each target gets one or two residues n mod S.
     const bool anyPar = __ballot(cls0 >= 2 || cls1 >= 2) != 0;
     const uint32_t S = s == 0 ? 2u : ((s & 1) && anyPar) ? 2 * s : s;
-    auto classes = [&](int cls, V yv, uint32_t (&rr)[2]) -> int {
+    auto classes = [&](int cls, V yav, V ybv, uint32_t (&rr)[2]) -> int {
       if (!cls) return 0;
       if (s == 0) {
         if (cls == 1) {
@@ -573,7 +623,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
         rr[0] = (uint32_t)(cls - 2);
         return 1;
       }
-      const int xl = dlog(yv);
+      const int xl = dlog(yav, ybv);
       if (xl < 0) return 0;
       const uint32_t ux = (uint32_t)xl;
       if (S == s) {
@@ -590,14 +640,21 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
       return 1;
     };
     uint32_t ra[2] = {0, 0}, rb[2] = {0, 0};
-    const int na = classes(cls0, y0, ra), nb = classes(cls1, y1, rb);
+    const int na = classes(cls0, ya0, yb0, ra), nb = classes(cls1, ya1, yb1, rb);
     const uint32_t nmod = job->nmin % S;
-    // b^-d for the shifts d < S <= 480: lane j holds b^-j (j < 64) and
-    // b^-(64i) for i = j mod 8, so b^-d is two shuffles and one product
-    // (taken with every lane active, outside the divergent stores)
+    // b^-d for the shifts d < S <= 10080: lane j holds b^-j and b^-(64j)
+    // (j < 64), and b^-(4096k) is one of three values, so b^-d is two
+    // shuffles, a select and two products (taken with every lane active,
+    // outside the divergent stores)
     const V bl = ar.pow_u64(binv, lane);
-    const V bh = ar.pow_u64(ar.mul(shfl64(bl, 63), binv), lane & 7u);
-    auto binv_pow = [&](uint32_t d) { return ar.mul(shfl64(bl, (int)(d & 63u)), shfl64(bh, (int)(d >> 6))); };
+    const V bh = ar.pow_u64(ar.mul(shfl64(bl, 63), binv), lane);
+    const V b4k = ar.mul(shfl64(bh, 63), ar.mul(shfl64(bl, 63), binv));  // b^-4096
+    const V b8k = ar.mul(b4k, b4k);
+    auto binv_pow = [&](uint32_t d) {
+      const uint32_t k = d >> 12;
+      const V top = k == 0 ? vone : k == 1 ? b4k : b8k;
+      return ar.mul(ar.mul(shfl64(bl, (int)(d & 63u)), shfl64(bh, (int)((d >> 6) & 63u))), top);
+    };
     uint32_t off = 0;
 #pragma unroll
     for (int a = 0; a < 2; a++) {

@@ -1,7 +1,7 @@
 """The residue-class rule of the setup's class tables (RS_ALGO_CLASSES),
 checked by brute force.
 
-For L = gcd(p - 1, 240) and z = b^((p-1)/L), every n with b^n = h (mod p)
+For L = gcd(p - 1, 5040) and z = b^((p-1)/L), every n with b^n = h (mod p)
 satisfies z^n = y with y = h^((p-1)/L), so n = x (mod s) for the x < s = ord(z)
 with z^x = y, and there is no n when y is not a power of z.  The setup keeps
 the reference's own tests (setup.cl:59-87: the parity classes and the
@@ -64,19 +64,36 @@ def _class_residues(p, b, hs, clss):
     """(S, residues n mod S per target) as the setup computes them."""
     pm = p - 1
     v2 = (pm & -pm).bit_length() - 1
-    L = (1 << min(v2, 4)) * (3 if pm % 3 == 0 else 1) * (5 if pm % 5 == 0 else 1)
+    La = (1 << min(v2, 4)) * (5 if pm % 5 == 0 else 1)                       # gcd(p-1, 80)
+    Lb = (9 if pm % 9 == 0 else 3 if pm % 3 == 0 else 1) * (7 if pm % 7 == 0 else 1)  # gcd(p-1, 63)
+    L = La * Lb                                                                # gcd(p-1, 5040)
     z = pow(b, pm // L, p)
-    s = next((i for i in range(1, L + 1) if pow(z, i, p) == 1), 0)
+    # n mod the orders of z^Lb and z^La (each exact: the first i with z_a^i = 1),
+    # combined by CRT when they are coprime (always for a prime p); otherwise
+    # (a composite p) only the parity split
+    za, zb = pow(z, Lb, p), pow(z, La, p)
+    sa = next((i for i in range(1, La + 1) if pow(za, i, p) == 1), 0)
+    sb = next((i for i in range(1, Lb + 1) if pow(zb, i, p) == 1), 0)
+    s = sa * sb if sa and sb and math.gcd(sa, sb) == 1 else 0
     any_par = any(c >= 2 for c in clss)
     S = 2 if s == 0 else (2 * s if (s & 1) and any_par else s)
-    table = {pow(z, i, p): i for i in range(s)}
+    ta = {pow(za, i, p): i for i in range(sa)} if s else {}
+    tb = {pow(zb, i, p): i for i in range(sb)} if s else {}
+
+    def dlog(h):
+        """x < s with z^x = y (its residues mod sa and mod sb), or None."""
+        y = pow(h, pm // L, p)
+        xa, xb = ta.get(pow(y, Lb, p)), tb.get(pow(y, La, p))
+        if xa is None or xb is None:
+            return None
+        return next(x for x in range(s) if x % sa == xa and x % sb == xb)
     res = []
     for h, cls in zip(hs, clss):
         rr = []
         if cls and s == 0:
             rr = [0, 1] if cls == 1 else [cls - 2]
         elif cls:
-            x = table.get(pow(h, pm // L, p))
+            x = dlog(h)
             if x is not None:
                 if S == s:
                     if not (cls >= 2 and (x & 1) != cls - 2):
