@@ -526,15 +526,23 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
 
   // Constants of the BSGS kernel, lane-parallel:
   //  lanes 0, 3, 4, 5: b^NMIN, b^T, b^(2T), b^(Qref-1)  ->  b^-1 = b^(Qref-1) * b^-Qref
-  V y = vone;
+  // With the residue classes b^(Qref-1) is taken on its own and b^NMIN joins
+  // the classes' constants round below, so this round is skipped.
+  V y = vone, bn = vone, bT0 = vone, bT1 = vone, binv;
   uint32_t ye = 0;
-  if (lane == 0) ye = job->nmin;
-  else if (lane == 3) ye = T;
-  else if (lane == 4) ye = 2 * T;
-  else if (lane == 5) ye = job->Qref - 1;
-  if (lane == 0 || (lane >= 3 && lane <= 5)) y = ar.pow_sm(mbase, ye);
-  const V bn = shfl64(y, 0), bT0 = shfl64(y, 3), bT1 = shfl64(y, 4);
-  const V binv = ar.mul(shfl64(y, 5), gQref_inv);
+  if (!split) {
+    if (lane == 0) ye = job->nmin;
+    else if (lane == 3) ye = T;
+    else if (lane == 4) ye = 2 * T;
+    else if (lane == 5) ye = job->Qref - 1;
+    if (lane == 0 || (lane >= 3 && lane <= 5)) y = ar.pow_sm(mbase, ye);
+    bn = shfl64(y, 0);
+    bT0 = shfl64(y, 3);
+    bT1 = shfl64(y, 4);
+    binv = ar.mul(shfl64(y, 5), gQref_inv);
+  } else {
+    binv = ar.mul(ar.base_pow(mbase, job->Qref - 1, base), gQref_inv);
+  }
 
   uint32_t nk[3] = {0, 0, 0};
   uint32_t cS = 0, cE = 0, cM = 0;  // residue-class shape of this prime
@@ -707,8 +715,8 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     cS = S;
     cE = (uint32_t)E;
     cM = (uint32_t)M;
-    // giant step b^-(S*E), its T-th power, the baby stride b^(S*T) and the
-    // table stride b^S (kept in bT[1])
+    // giant step b^-(S*E), its T-th power, the baby stride b^(S*T), the
+    // table stride b^S (kept in bT[1]) and b^NMIN
     uint64_t ye2 = 0;
     V yb2 = binv;
     if (lane == 0) ye2 = SE;
@@ -719,13 +727,17 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     } else if (lane == 3) {
       ye2 = S;
       yb2 = mbase;
+    } else if (lane == 4) {
+      ye2 = job->nmin;
+      yb2 = mbase;
     }
     V y2 = vone;
-    if (lane < 4) y2 = ar.pow_u64(yb2, ye2);
+    if (lane < 5) y2 = ar.pow_u64(yb2, ye2);
     g0v = g1v = shfl64(y2, 0);
     gT0v = gT1v = shfl64(y2, 1);
     bTa = shfl64(y2, 2);
     bTb = shfl64(y2, 3);
+    bn = shfl64(y2, 4);
   } else {
     //  lanes 0, 1, 2: b^-Q, b^-QT, b^-2QT (giant step of the table span Q)
     ye = 0;
