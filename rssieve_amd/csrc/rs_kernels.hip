@@ -269,23 +269,41 @@ __device__ __forceinline__ void fp_powmod_lg2(double x0, double x1, uint64_t exp
 // d of the power-residue tests (setup.cl:22-48)
 __constant__ uint32_t g_prd[12] = {3, 4, 5, 7, 8, 9, 11, 13, 17, 19, 23, 29};
 
-// Residue classes of n (algo 2).  With L = gcd(p - 1, 5040) and z =
-// b^((p-1)/L), every solution of b^n = h (mod p) satisfies z^n = y with y =
-// h^((p-1)/L).  Split L = La * Lb with La = gcd(p - 1, 80) and Lb =
-// gcd(p - 1, 63): for za = z^Lb of order sa (the first i > 0 with za^i = 1)
-// the powers za^i, i < sa, are distinct, so ya = y^Lb = za^xa fixes
-// n = xa (mod sa), and likewise n = xb (mod sb) from zb = z^La, yb = y^La;
-// when sa and sb are coprime (always for a prime p) the two combine by CRT
-// into n mod s, s = sa*sb = ord(z), and a y that is no power of z leaves no n
+// Residue classes of n (algo 2).  With L = gcd(p - 1, 720720) (720720 =
+// 2^4 * 3^2 * 5 * 7 * 11 * 13) and z = b^((p-1)/L), every solution of b^n = h
+// (mod p) satisfies z^n = y with y = h^((p-1)/L).  Split L = La * Lb * Lc
+// with La = gcd(p - 1, 80), Lb = gcd(p - 1, 63) and Lc = gcd(p - 1, 143): for
+// za = z^(Lb*Lc) of order sa (the first i > 0 with za^i = 1) the powers za^i,
+// i < sa, are distinct, so ya = y^(Lb*Lc) = za^xa fixes n = xa (mod sa), and
+// likewise n = xb (mod sb) from zb = z^(La*Lc), yb = y^(La*Lc), and n = xc
+// (mod sc) from zc = z^(La*Lb), yc = y^(La*Lb); when sa, sb and sc are
+// pairwise coprime (always for a prime p) the three combine by CRT into
+// n mod s, s = sa*sb*sc = ord(z), and a y that is no power of z leaves no n
 // at all.  This is algebra mod p, true for a composite 2-PRP as well (whose
 // non-coprime orders fall back to the parity split): the reference's tests
 // (parity classes, power residues, setup.cl:59-87) still decide which k are
 // searched, and the classes only narrow the n they are searched on, so the
 // solutions found are the same.  For a prime p with b a quadratic
 // non-residue s is even and fixes the parity the reference's class gives.
-constexpr uint32_t kZV = 256;                 // za^i at [0, 81), zb^i at [128, 192)
-constexpr uint32_t kZH = 512;                 // their hash slots: [0, 256) and [256, 384)
-constexpr uint64_t kClassMinP = 16384;        // smaller p keep the merged table's parity split (exponent (p-1)/L >= 2)
+constexpr uint32_t kZV = 336;                 // za^i at [0, 81), zb^i at [128, 192), zc^i at [192, 336)
+constexpr uint32_t kZH = 640;                 // their hash slots: [0, 256), [256, 384) and [384, 640)
+constexpr uint64_t kClassMinP = 1u << 20;     // smaller p keep the merged table's parity split; above it
+                                              // the exponent (p-1)/L is at least 2 (L <= 720720)
+
+// a^-1 mod m for gcd(a, m) = 1 and 1 <= m < 2^31 (extended Euclid); 0 for m = 1
+__device__ __forceinline__ uint32_t inv_mod_small(uint32_t a, uint32_t m) {
+  int r0 = (int)m, r1 = (int)(a % m), t0 = 0, t1 = 1;
+  while (r1) {
+    const int qq = r0 / r1;
+    int t = r0 - qq * r1;
+    r0 = r1;
+    r1 = t;
+    t = t0 - qq * t1;
+    t0 = t1;
+    t1 = t;
+  }
+  return t0 < 0 ? (uint32_t)(t0 + (int)m) : (uint32_t)t0;
+}
 
 // LDS operations of one wavefront become visible to its other lanes
 __device__ __forceinline__ void wave_sync() {
@@ -399,16 +417,17 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   const V AB = ar.mul(A0, B0);
   const V total = ar.mul(AB, gQ);  // prefix[KCOUNT] of setup.cl:147-151
 
-  // residue classes: L = La * Lb = gcd(p - 1, 80) * gcd(p - 1, 63)
+  // residue classes: L = La * Lb * Lc = gcd(p - 1, 80) * gcd(p - 1, 63) * gcd(p - 1, 143)
   const uint64_t pm = p - 1;
   const bool split = algo == 2 && p > kClassMinP;
-  uint32_t L = 2, La = 2, Lb = 1;
+  uint32_t L = 2, La = 2, Lb = 1, Lc = 1;
   uint64_t exL = 0;
   if (split) {
     const uint32_t v2 = (uint32_t)__builtin_ctzll(pm);
     La = (1u << (v2 < 4 ? v2 : 4)) * (pm % 5 == 0 ? 5u : 1u);
     Lb = (pm % 9 == 0 ? 9u : pm % 3 == 0 ? 3u : 1u) * (pm % 7 == 0 ? 7u : 1u);
-    L = La * Lb;
+    Lc = (pm % 11 == 0 ? 11u : 1u) * (pm % 13 == 0 ? 13u : 1u);
+    L = La * Lb * Lc;
     exL = pm / L;
   }
 
@@ -471,13 +490,21 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   // the character h^((p-1)/d) is y^(L/d) (the same value from a short
   // exponent): the reference's quadratic test and its power-residue tests
   // with d | L take it from y.
-  V y0 = vone, y1 = vone, ya0 = vone, ya1 = vone, yb0 = vone, yb1 = vone;
+  V y0 = vone, y1 = vone, ya0 = vone, ya1 = vone, yb0 = vone, yb1 = vone, yc0 = vone, yc1 = vone;
   if (split) {
     ar.pow_lg2(h0, h1, exL, &y0, &y1);
-    ya0 = ar.pow_u64(y0, Lb);
-    ya1 = ar.pow_u64(y1, Lb);
-    yb0 = ar.pow_u64(y0, La);
-    yb1 = ar.pow_u64(y1, La);
+    // u = y^Lc, ya = u^Lb, yb = u^La, yc = y^(La*Lb); Lc = 1 for most p
+    V u0 = y0, u1 = y1;
+    if (Lc > 1) {
+      u0 = ar.pow_u64(y0, Lc);
+      u1 = ar.pow_u64(y1, Lc);
+      yc0 = ar.pow_u64(y0, La * Lb);
+      yc1 = ar.pow_u64(y1, La * Lb);
+    }
+    ya0 = ar.pow_u64(u0, Lb);
+    ya1 = ar.pow_u64(u1, Lb);
+    yb0 = ar.pow_u64(u0, La);
+    yb1 = ar.pow_u64(u1, La);
   }
   auto sign_of = [&](V x0, V x1, uint32_t e, int* t0, int* t1) {
     const V a0 = ar.pow_u64(x0, e), a1 = ar.pow_u64(x1, e);
@@ -501,7 +528,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   {
     int t0, t1;
     if (split) {
-      sign_of(ya0, ya1, La / 2, &t0, &t1);  // y^(L/2) = (y^Lb)^(La/2), Lb odd
+      sign_of(ya0, ya1, La / 2, &t0, &t1);  // y^(L/2) = (y^(Lb*Lc))^(La/2), Lb*Lc odd
     } else {
       pow_sign2(pm >> 1, &t0, &t1);
     }
@@ -549,35 +576,38 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   V g0v, g1v, gT0v, gT1v, bTa, bTb;
   KRec* kr = J.krecs + (size_t)pi * (merged ? 2 * K : K);
   if (split) {
-    // the tables of za^i (i <= La, at zv[0..]) and zb^i (i <= Lb, at
-    // zv[128..]), and their orders sa, sb (0: no i > 0 with value 1 in range,
-    // possible only for a composite p)
+    // the tables of za^i (i <= La, at zv[0..]), zb^i (i <= Lb, at zv[128..])
+    // and zc^i (i <= Lc, at zv[192..]), and their orders sa, sb, sc (0: no
+    // i > 0 with value 1 in range, possible only for a composite p)
     for (uint32_t h = lane; h < kZH; h += 64) zh[h] = 0u;
-    const V za = ar.pow_u64(zr, Lb), zb = ar.pow_u64(zr, La);
-    uint32_t sa = 0, sb = 0;
-    {
-      V cur = ar.pow_u64(za, lane);
-      const V z64 = ar.mul(shfl64(cur, 63), za);
-      for (uint32_t j = 0; 64 * j <= La; j++) {
+    const V za = ar.pow_u64(zr, Lb * Lc), zb = ar.pow_u64(zr, La * Lc), zc = Lc > 1 ? ar.pow_u64(zr, La * Lb) : vone;
+    // z^i for i <= Lz at zv[vb + i], 64 per pass; the order of z in range
+    auto table = [&](V zz, uint32_t Lz, uint32_t vb) -> uint32_t {
+      uint32_t sz = 0;
+      V cur = ar.pow_u64(zz, lane);
+      const V z64 = ar.mul(shfl64(cur, 63), zz);
+      for (uint32_t j = 0; 64 * j <= Lz; j++) {
         const uint32_t idx = 64 * j + lane;
-        const bool in = idx <= La;
-        if (in) zv[idx] = ar.bits(cur);
+        const bool in = idx <= Lz;
+        if (in) zv[vb + idx] = ar.bits(cur);
         const uint64_t m1 = __ballot(in && idx > 0 && cur == vone);
-        if (!sa && m1) sa = 64 * j + (uint32_t)__ffsll((long long)m1) - 1;
-        cur = ar.mul(cur, z64);
+        if (!sz && m1) sz = 64 * j + (uint32_t)__ffsll((long long)m1) - 1;
+        if (64 * (j + 1) <= Lz) cur = ar.mul(cur, z64);
       }
-      const V cb = ar.pow_u64(zb, lane);  // lane <= 63 covers i <= Lb
-      if (lane <= Lb) zv[128 + lane] = ar.bits(cb);
-      const uint64_t m1 = __ballot(lane <= Lb && lane > 0 && cb == vone);
-      if (m1) sb = (uint32_t)__ffsll((long long)m1) - 1;
-    }
-    uint32_t g = sa, h2 = sb;  // gcd(sa, sb)
-    while (h2) {
-      const uint32_t t = g % h2;
-      g = h2;
-      h2 = t;
-    }
-    const uint32_t s = (sa && sb && g == 1) ? sa * sb : 0u;
+      return sz;
+    };
+    // (Lc = 1 for most p: then sc = 1 and the third table is not used)
+    const uint32_t sa = table(za, La, 0u), sb = table(zb, Lb, 128u), sc = Lc > 1 ? table(zc, Lc, 192u) : 1u;
+    auto gcd = [](uint32_t g, uint32_t h2) {
+      while (h2) {
+        const uint32_t t = g % h2;
+        g = h2;
+        h2 = t;
+      }
+      return g;
+    };
+    const uint32_t sab = sa * sb;
+    const uint32_t s = (sa && sb && sc && gcd(sa, sb) == 1 && gcd(sab, sc) == 1) ? sab * sc : 0u;
     wave_sync();
     if (s) {
       for (uint32_t idx = lane; idx < sa; idx += 64) {
@@ -588,16 +618,14 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
         uint32_t h = (uint32_t)zv[128 + lane] & 127u;
         while (atomicCAS(&zh[256 + h], 0u, lane + 1) != 0u) h = (h + 1) & 127u;
       }
+      for (uint32_t idx = lane; Lc > 1 && idx < sc; idx += 64) {
+        uint32_t h = (uint32_t)zv[192 + idx] & 255u;
+        while (atomicCAS(&zh[384 + h], 0u, idx + 1) != 0u) h = (h + 1) & 255u;
+      }
     }
     wave_sync();
-    // sa^-1 mod sb for the CRT (sb <= 63)
-    uint32_t sainv = 0;
-    if (s)
-      for (uint32_t t = 0; t < sb; t++)
-        if ((sa * t) % sb == 1u % sb) {
-          sainv = t;
-          break;
-        }
+    // sa^-1 mod sb and (sa*sb)^-1 mod sc for the CRT
+    const uint32_t sainv = s ? inv_mod_small(sa, sb) : 0u, sabinv = s ? inv_mod_small(sab, sc) : 0u;
     // index of a key in one table (hash base hb, mask hm, values at vb), or -1
     auto lookup = [&](uint64_t key, uint32_t hb, uint32_t hm, uint32_t vb) -> int {
       for (uint32_t h = (uint32_t)key & hm;; h = (h + 1) & hm) {
@@ -606,21 +634,29 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
         if (zv[vb + sl - 1] == key) return (int)sl - 1;
       }
     };
-    // the x < s with z^x = y, from ya = y^Lb and yb = y^La, or -1
-    auto dlog = [&](V yav, V ybv) -> int {
+    // the x < s with z^x = y, from ya = y^(Lb*Lc), yb = y^(La*Lc) and
+    // yc = y^(La*Lb), or -1
+    auto dlog = [&](V yav, V ybv, V ycv) -> int {
       const int xa = lookup(ar.bits(yav), 0u, 255u, 0u);
       if (xa < 0) return -1;
       const int xb = lookup(ar.bits(ybv), 256u, 127u, 128u);
       if (xb < 0) return -1;
+      int xc = 0;
+      if (Lc > 1) {
+        xc = lookup(ar.bits(ycv), 384u, 255u, 192u);
+        if (xc < 0) return -1;
+      }
       const uint32_t dx = ((uint32_t)xb + sb - (uint32_t)xa % sb) % sb;
-      return xa + (int)(sa * ((dx * sainv) % sb));
+      const uint32_t xab = (uint32_t)xa + sa * ((dx * sainv) % sb);
+      const uint32_t dy = ((uint32_t)xc + sc - xab % sc) % sc;
+      return (int)(xab + sab * ((dy * sabinv) % sc));
     };
     // Table stride S: s when s is even (it fixes the parity) or no k has a
     // parity class here, else 2s (CRT of the class mod s and the parity);
     // 2 without a usable s.  Each target gets one or two residues n mod S.
     const bool anyPar = __ballot(cls0 >= 2 || cls1 >= 2) != 0;
     const uint32_t S = s == 0 ? 2u : ((s & 1) && anyPar) ? 2 * s : s;
-    auto classes = [&](int cls, V yav, V ybv, uint32_t (&rr)[2]) -> int {
+    auto classes = [&](int cls, V yav, V ybv, V ycv, uint32_t (&rr)[2]) -> int {
       if (!cls) return 0;
       if (s == 0) {
         if (cls == 1) {
@@ -631,7 +667,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
         rr[0] = (uint32_t)(cls - 2);
         return 1;
       }
-      const int xl = dlog(yav, ybv);
+      const int xl = dlog(yav, ybv, ycv);
       if (xl < 0) return 0;
       const uint32_t ux = (uint32_t)xl;
       if (S == s) {
@@ -648,20 +684,26 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
       return 1;
     };
     uint32_t ra[2] = {0, 0}, rb[2] = {0, 0};
-    const int na = classes(cls0, ya0, yb0, ra), nb = classes(cls1, ya1, yb1, rb);
+    const int na = classes(cls0, ya0, yb0, yc0, ra), nb = classes(cls1, ya1, yb1, yc1, rb);
     const uint32_t nmod = job->nmin % S;
-    // b^-d for the shifts d < S <= 10080: lane j holds b^-j and b^-(64j)
-    // (j < 64), and b^-(4096k) is one of three values, so b^-d is two
-    // shuffles, a select and two products (taken with every lane active,
-    // outside the divergent stores)
+    // b^-d for the shifts d < S <= 1441440 < 2^21, in base-64 digits: lane j
+    // holds b^-j, b^-(64j), b^-(4096j) and b^-(262144j) (j < 64; the last two
+    // levels only when S needs them), so b^-d is one shuffle per digit and a
+    // product between (taken with every lane active, outside the divergent
+    // stores)
     const V bl = ar.pow_u64(binv, lane);
-    const V bh = ar.pow_u64(ar.mul(shfl64(bl, 63), binv), lane);
-    const V b4k = ar.mul(shfl64(bh, 63), ar.mul(shfl64(bl, 63), binv));  // b^-4096
-    const V b8k = ar.mul(b4k, b4k);
+    const V b64 = ar.mul(shfl64(bl, 63), binv);  // b^-64
+    const V bh = ar.pow_u64(b64, lane);
+    V b3 = vone, b4 = vone;
+    if (S > 4096) {
+      b3 = ar.pow_u64(ar.mul(shfl64(bh, 63), b64), lane);  // (b^-4096)^j
+      if (S > 262144) b4 = ar.pow_u64(ar.mul(shfl64(b3, 63), ar.mul(shfl64(bh, 63), b64)), lane);
+    }
     auto binv_pow = [&](uint32_t d) {
-      const uint32_t k = d >> 12;
-      const V top = k == 0 ? vone : k == 1 ? b4k : b8k;
-      return ar.mul(ar.mul(shfl64(bl, (int)(d & 63u)), shfl64(bh, (int)((d >> 6) & 63u))), top);
+      V r = ar.mul(shfl64(bl, (int)(d & 63u)), shfl64(bh, (int)((d >> 6) & 63u)));
+      if (S > 4096) r = ar.mul(r, shfl64(b3, (int)((d >> 12) & 63u)));
+      if (S > 262144) r = ar.mul(r, shfl64(b4, (int)(d >> 18)));
+      return r;
     };
     uint32_t off = 0;
 #pragma unroll
@@ -1087,7 +1129,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
   extern __shared__ uint4 lds4[];
   uint32_t* tab = (uint32_t*)lds4;  // the filter; the bucketed table of the overflow recovery reuses it
   __shared__ uint64_t s_h[2 * RS_MAXK + 16];  // targets of the item (+ read-ahead pad)
-  __shared__ uint32_t s_k[2 * RS_MAXK];   // sequence index | shift << 16
+  static_assert(RS_MAXK <= 256, "a sequence index fits the low 8 bits of s_k");
+  __shared__ uint32_t s_k[2 * RS_MAXK];   // sequence index | shift (< 2^21) << 8
   __shared__ uint32_t s_item, s_job, s_order, s_dups, s_claims, s_qn;
   __shared__ uint32_t s_next;  // the block's next item when it is known (same chunk and job), else ~0
   __shared__ uint4 s_q[kQCap];  // filter positives of the item: residue, target, giant step
@@ -1271,7 +1314,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     // 61-63).
     auto claim_hit = [&](uint32_t e, uint32_t slot, uint32_t ord) {
       const uint4 qe = s_q[slot];
-      const uint32_t kk = s_k[qe.z], kidx = kk & 0xFFFFu, d = kk >> 16;
+      const uint32_t kk = s_k[qe.z], kidx = kk & 0xFFu, d = kk >> 8;
       if (ord) {
         for (uint64_t n = (uint64_t)nstart + stride * e + d; n <= nmax; n += ord)
           claim(job, bitmap, fac, cnt, p, kidx, (uint32_t)n, &s_dups, &s_claims);
@@ -1342,7 +1385,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         const int e = gtab_find_at(gtab, gbmask, mbase, p, q, one, base, nstart, stride, g);
         if (e >= 0) {
           const uint32_t kk = s_k[i];
-          claim(job, bitmap, fac, cnt, p, kk & 0xFFFFu, nstart + qi * Ql + stride * (uint32_t)e + (kk >> 16),
+          claim(job, bitmap, fac, cnt, p, kk & 0xFFu, nstart + qi * Ql + stride * (uint32_t)e + (kk >> 8),
                 &s_dups, &s_claims);
         }
       };
@@ -1384,7 +1427,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     }
     for (uint32_t i = tid; i < nk; i += T) {
       s_h[i] = kr[i].hadj;
-      s_k[i] = kr[i].kidx | ((merged ? kr[i].cls : 0u) << 16);
+      s_k[i] = kr[i].kidx | ((merged ? kr[i].cls : 0u) << 8);
     }
     if (tid == 0) s_order = 0xFFFFFFFFu;
     __syncthreads();

@@ -1,7 +1,7 @@
 """The residue-class rule of the setup's class tables (RS_ALGO_CLASSES),
 checked by brute force.
 
-For L = gcd(p - 1, 5040) and z = b^((p-1)/L), every n with b^n = h (mod p)
+For L = gcd(p - 1, 720720) and z = b^((p-1)/L), every n with b^n = h (mod p)
 satisfies z^n = y with y = h^((p-1)/L), so n = x (mod s) for the x < s = ord(z)
 with z^x = y, and there is no n when y is not a power of z.  The setup keeps
 the reference's own tests (setup.cl:59-87: the parity classes and the
@@ -66,27 +66,33 @@ def _class_residues(p, b, hs, clss):
     v2 = (pm & -pm).bit_length() - 1
     La = (1 << min(v2, 4)) * (5 if pm % 5 == 0 else 1)                       # gcd(p-1, 80)
     Lb = (9 if pm % 9 == 0 else 3 if pm % 3 == 0 else 1) * (7 if pm % 7 == 0 else 1)  # gcd(p-1, 63)
-    L = La * Lb                                                                # gcd(p-1, 5040)
+    Lc = (11 if pm % 11 == 0 else 1) * (13 if pm % 13 == 0 else 1)                    # gcd(p-1, 143)
+    L = La * Lb * Lc                                                           # gcd(p-1, 720720)
     z = pow(b, pm // L, p)
-    # n mod the orders of z^Lb and z^La (each exact: the first i with z_a^i = 1),
-    # combined by CRT when they are coprime (always for a prime p); otherwise
-    # (a composite p) only the parity split
-    za, zb = pow(z, Lb, p), pow(z, La, p)
+    # n mod the orders of z^(Lb*Lc), z^(La*Lc) and z^(La*Lb) (each exact: the
+    # first i with z_a^i = 1), combined by CRT when they are pairwise coprime
+    # (always for a prime p); otherwise (a composite p) only the parity split
+    za, zb, zc = pow(z, Lb * Lc, p), pow(z, La * Lc, p), pow(z, La * Lb, p)
     sa = next((i for i in range(1, La + 1) if pow(za, i, p) == 1), 0)
     sb = next((i for i in range(1, Lb + 1) if pow(zb, i, p) == 1), 0)
-    s = sa * sb if sa and sb and math.gcd(sa, sb) == 1 else 0
+    # Lc = 1: no constraint mod sc = 1 and no third table (as the kernel)
+    sc = 1 if Lc == 1 else next((i for i in range(1, Lc + 1) if pow(zc, i, p) == 1), 0)
+    coprime = math.gcd(sa, sb) == 1 and math.gcd(sa * sb, sc) == 1
+    s = sa * sb * sc if sa and sb and sc and coprime else 0
     any_par = any(c >= 2 for c in clss)
     S = 2 if s == 0 else (2 * s if (s & 1) and any_par else s)
     ta = {pow(za, i, p): i for i in range(sa)} if s else {}
     tb = {pow(zb, i, p): i for i in range(sb)} if s else {}
+    tc = {pow(zc, i, p): i for i in range(sc)} if s and Lc > 1 else {}
 
     def dlog(h):
-        """x < s with z^x = y (its residues mod sa and mod sb), or None."""
+        """x < s with z^x = y (its residues mod sa, sb and sc), or None."""
         y = pow(h, pm // L, p)
-        xa, xb = ta.get(pow(y, Lb, p)), tb.get(pow(y, La, p))
-        if xa is None or xb is None:
+        xa, xb = ta.get(pow(y, Lb * Lc, p)), tb.get(pow(y, La * Lc, p))
+        xc = tc.get(pow(y, La * Lb, p)) if Lc > 1 else 0
+        if xa is None or xb is None or xc is None:
             return None
-        return next(x for x in range(s) if x % sa == xa and x % sb == xb)
+        return next(x for x in range(s) if x % sa == xa and x % sb == xb and x % sc == xc)
     res = []
     for h, cls in zip(hs, clss):
         rr = []
