@@ -389,7 +389,8 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   // reference's own split for this n-range (setup.cl:137-162): for a composite
   // 2-PRP the test's outcome depends on that exponent, so every BSGS shape
   // uses the same one and skips the same numbers.
-  const V gQ = ar.base_pow(mbase, job->Qref, base);
+  const V bQ1 = ar.base_pow(mbase, job->Qref - 1, base);  // b^(Qref-1), for b^-1 below
+  const V gQ = ar.mul(bQ1, mbase);
 
   // inclusive prefix (s) and suffix (u) products over the wavefront
   V s0 = mk0, s1 = mk1, u0 = mk0, u1 = mk1;
@@ -435,7 +436,9 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   //  lane 0: Fermat inverse total^(p-2)          (setup.cl:154)
   //  lanes 1..12: base^((p-1)/d) for d | p-1       (setup.cl:22-48)
   //  lane 13: base^((p-1)/2)                       (setup.cl:51-53)
-  //  lane 14: z = base^((p-1)/L) of the residue classes
+  //  lanes 14, 15, 16: the residue classes' za = base^((p-1)/La),
+  //    zb = base^((p-1)/Lb) and zc = base^((p-1)/Lc) (with Lc > 1)
+  //  lane 17: base^NMIN (residue classes)
   uint64_t ex = 0;
   V x = mbase;
   bool dvalid = false;
@@ -449,16 +452,22 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     if (!dvalid) ex = 0;
   } else if (lane == 13) {
     ex = pm >> 1;
-  } else if (lane == 14) {
-    ex = exL;
+  } else if (split && lane == 14) {
+    ex = pm / La;
+  } else if (split && lane == 15) {
+    ex = pm / Lb;
+  } else if (split && lane == 16 && Lc > 1) {
+    ex = pm / Lc;
+  } else if (split && lane == 17) {
+    ex = job->nmin;
   }
   V r = vone;
   if (ex) r = ar.pow_lg(x, ex);
+  if (ex == 1 && lane == 17) r = mbase;  // the masked shift of pow_lg needs an exponent >= 2
   const V inv_total = shfl64(r, 0);
   const bool invertible = ar.mul(total, inv_total) == vone;  // setup.cl:156-162
   const uint64_t dmask = __ballot(lane <= 12 && dvalid && r == vone) >> 1;  // passing d tests, bit i = g_prd[i]
   const V rg = shfl64(r, 13);
-  const V zr = shfl64(r, 14);
   const int resg = (rg == vone) - (rg == nmo);
 
   if (!invertible) {
@@ -552,24 +561,21 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   const bool merged = algo != 0;
 
   // Constants of the BSGS kernel, lane-parallel:
-  //  lanes 0, 3, 4, 5: b^NMIN, b^T, b^(2T), b^(Qref-1)  ->  b^-1 = b^(Qref-1) * b^-Qref
-  // With the residue classes b^(Qref-1) is taken on its own and b^NMIN joins
-  // the classes' constants round below, so this round is skipped.
+  //  lanes 0, 3, 4: b^NMIN, b^T, b^(2T); b^-1 = b^(Qref-1) * b^-Qref
+  // With the residue classes b^NMIN came with the first round and the
+  // classes' constants follow below, so this round is skipped.
   V y = vone, bn = vone, bT0 = vone, bT1 = vone, binv;
   uint32_t ye = 0;
   if (!split) {
     if (lane == 0) ye = job->nmin;
     else if (lane == 3) ye = T;
     else if (lane == 4) ye = 2 * T;
-    else if (lane == 5) ye = job->Qref - 1;
-    if (lane == 0 || (lane >= 3 && lane <= 5)) y = ar.pow_sm(mbase, ye);
+    if (lane == 0 || lane == 3 || lane == 4) y = ar.pow_sm(mbase, ye);
     bn = shfl64(y, 0);
     bT0 = shfl64(y, 3);
     bT1 = shfl64(y, 4);
-    binv = ar.mul(shfl64(y, 5), gQref_inv);
-  } else {
-    binv = ar.mul(ar.base_pow(mbase, job->Qref - 1, base), gQref_inv);
   }
+  binv = ar.mul(bQ1, gQref_inv);
 
   uint32_t nk[3] = {0, 0, 0};
   uint32_t cS = 0, cE = 0, cM = 0;  // residue-class shape of this prime
@@ -580,7 +586,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     // and zc^i (i <= Lc, at zv[192..]), and their orders sa, sb, sc (0: no
     // i > 0 with value 1 in range, possible only for a composite p)
     for (uint32_t h = lane; h < kZH; h += 64) zh[h] = 0u;
-    const V za = ar.pow_u64(zr, Lb * Lc), zb = ar.pow_u64(zr, La * Lc), zc = Lc > 1 ? ar.pow_u64(zr, La * Lb) : vone;
+    const V za = shfl64(r, 14), zb = shfl64(r, 15), zc = Lc > 1 ? shfl64(r, 16) : vone;
     // z^i for i <= Lz at zv[vb + i], 64 per pass; the order of z in range
     auto table = [&](V zz, uint32_t Lz, uint32_t vb) -> uint32_t {
       uint32_t sz = 0;
@@ -757,29 +763,25 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     cS = S;
     cE = (uint32_t)E;
     cM = (uint32_t)M;
-    // giant step b^-(S*E), its T-th power, the baby stride b^(S*T), the
-    // table stride b^S (kept in bT[1]) and b^NMIN
+    // the giant step b^-(S*E) and the table stride b^S (kept in bT[1]) by
+    // lanes 0 and 1, then their T-th powers by log2(T) squarings: the giant
+    // step of a workgroup round b^-(S*E*T) and the baby stride b^(S*T)
     uint64_t ye2 = 0;
     V yb2 = binv;
     if (lane == 0) ye2 = SE;
-    else if (lane == 1) ye2 = SE * T;
-    else if (lane == 2) {
-      ye2 = (uint64_t)S * T;
-      yb2 = mbase;
-    } else if (lane == 3) {
+    else if (lane == 1) {
       ye2 = S;
-      yb2 = mbase;
-    } else if (lane == 4) {
-      ye2 = job->nmin;
       yb2 = mbase;
     }
     V y2 = vone;
-    if (lane < 5) y2 = ar.pow_u64(yb2, ye2);
+    if (lane < 2) y2 = ar.pow_u64(yb2, ye2);
+    V y2T = y2;
+    for (uint32_t t = 1; t < T; t <<= 1) y2T = ar.mul(y2T, y2T);
     g0v = g1v = shfl64(y2, 0);
-    gT0v = gT1v = shfl64(y2, 1);
-    bTa = shfl64(y2, 2);
-    bTb = shfl64(y2, 3);
-    bn = shfl64(y2, 4);
+    gT0v = gT1v = shfl64(y2T, 0);
+    bTa = shfl64(y2T, 1);
+    bTb = shfl64(y2, 1);
+    bn = shfl64(r, 17);
   } else {
     //  lanes 0, 1, 2: b^-Q, b^-QT, b^-2QT (giant step of the table span Q)
     ye = 0;
@@ -901,7 +903,9 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   if (lane == 0) *R = rec;
 }
 
-__global__ __launch_bounds__(256) void k_setup(SegJob* __restrict__ jobs) {
+// five wavefronts per SIMD (at most 96 VGPRs): the setup is issue-bound and
+// needs the waves to cover its FP64 and LDS latencies
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_setup(SegJob* __restrict__ jobs) {
   const SegJob& J = jobs[blockIdx.y];
   Counters* __restrict__ cnt = J.cnt;
   const uint32_t cap = J.cap;
