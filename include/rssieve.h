@@ -40,7 +40,7 @@ enum {
   RS_ALGO_THREE_LISTS = 0, /* reference shape: full / even / odd k lists, one table each (giant.cl) */
   RS_ALGO_MERGED = 1,      /* one table of even exponents shared by all classes */
   RS_ALGO_CLASSES = 2      /* one table per prime of stride S: the class of n mod S each target's
-                              power-residue character fixes (S | 2 * gcd(p - 1, 240)); Q is the
+                              power-residue character fixes (S | 2 * gcd(p - 1, 720720)); Q is the
                               largest table, in entries; the eliminated terms are the same */
 };
 
