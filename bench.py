@@ -308,7 +308,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=16, help="views of the job sieved per batched round")
+    ap.add_argument("--batch", type=int, default=32, help="views of the job sieved per batched round")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--tuned-only", action="store_true", help="skip the plain leg (for profiling the tuned kernel)")

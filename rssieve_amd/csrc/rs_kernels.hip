@@ -1415,7 +1415,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         redo(r0);
         return true;
       }
-      if (final || 2 * nq >= qcap) {
+      if (nq && (final || 2 * nq >= qcap)) {  // no positive, nothing to match
         match(nq, nent, 0);
         __syncthreads();
         if (tid == 0) s_qn = 0;
