@@ -1038,18 +1038,21 @@ __device__ __noinline__ int gtab_find_at(const uint32_t* tab, uint32_t bmask, ui
   }
 }
 
-// Blocked Bloom filter of the baby steps: hsize 32-bit words in LDS, three
+// Blocked Bloom filter of the baby steps: hsize 32-bit words in LDS, four
 // bits per baby value, all in the word chosen by the residue's low bits.
-// With hsize/2 values the false-positive rate is about 9e-4 (two bits gave
-// 2.8e-3), with hsize values about 3e-3, so a giant step costs one
-// ds_read_b32 and a few integer ops; the rare positives are queued and
-// resolved exactly.  Residues of products mod p are spread evenly over their
-// low bits: bits 0..14 pick the word (hsize <= 32768), bits 16..20, 24..28
-// (byte-aligned fields, one shift each with a sub-dword operand select) and
-// 32..36 the three bits.
+// With hsize/2 values the false-positive rate is about 4.7e-4 (three bits
+// gave 9.9e-4, two 2.8e-3), with hsize values about 1.9e-3 (three bits:
+// 3e-3), so a giant step costs one ds_read_b32 and a few integer ops; the
+// rare positives are queued and resolved exactly, and the fewer there are,
+// the less often a wavefront takes the queueing branch (whose returning LDS
+// atomic waits for every LDS read in flight).  Residues of products mod p are
+// spread evenly over their low bits: bits 0..14 pick the word (hsize <=
+// 32768), bits 16..20, 24..28, 32..36 and 40..44 the four bits (for p below
+// 2^45 the last fields are partly zero and their bits coincide, which only
+// costs selectivity).
 __device__ __forceinline__ uint32_t flt_bits(uint64_t v) {
   const uint32_t x = (uint32_t)v, y = (uint32_t)(v >> 32);
-  return (1u << ((x >> 16) & 31u)) | (1u << ((x >> 24) & 31u)) | (1u << (y & 31u));
+  return (1u << ((x >> 16) & 31u)) | (1u << ((x >> 24) & 31u)) | (1u << (y & 31u)) | (1u << ((y >> 8) & 31u));
 }
 __device__ __forceinline__ void flt_insert(uint32_t* flt, uint32_t fmask, uint64_t v) {
   atomicOr(&flt[(uint32_t)v & fmask], flt_bits(v));
@@ -1242,9 +1245,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     // Giant steps run in F = Ml / T full rounds (q = tid + r*T) and a last
     // partial round of m = Ml - F*T steps spread over all threads.
     const uint32_t F = Ml / T, m = Ml - F * T;
-    // Rounds between queue checks: a round queues about T * nk * 9e-4
+    // Rounds between queue checks: a round queues about T * nk * 4.7e-4
     // positives (the filter's rate at half load); checking when a quarter of
-    // the queue could be filled at 2e-3 keeps overflows rare without a
+    // the queue could be filled at 2e-3 (one value per word) keeps overflows rare without a
     // barrier per round.
     const uint32_t every = max(1u, 125u * qcap / (T * nk));
 
