@@ -317,8 +317,8 @@ def main():
                     help="first copy this many MiB device to device (a kernel of known traffic for PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the sieve shards by p-range, "
-              "see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU; --gpus %d is not supported (several GPUs would each sieve their own "
+              "p-range and merge the cleared terms at the end)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.steps + args.warmup > (PRANGE - 2**31) // SLICE:
         sys.exit("bench.py: too many steps for the p-range")
