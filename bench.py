@@ -20,7 +20,9 @@ Two paths sieve the same slices of the same terms:
 Their final bitmaps must be identical (checked).  'value' is the tuned path.
 A third leg ('jobs16') sieves sixteen uneven synthetic jobs one at a time
 with the reference split and then all in one batch with the cost model's
-picks, and compares the bitmaps.
+picks, and compares the bitmaps.  With --other-workloads, the small case
+(b=2, K=10, p from 10^9) and the high-p case (b=5, K=100, p from 2^63) are
+timed too, one job each, reference split against the cost model's pick.
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -204,6 +206,81 @@ def choose_config(ctx, job, data, log):
     return cands[idx], how, idx
 
 
+def _small_case():
+    """The reference's CPU-sized case: b=2, ten odd k in [3, 10^6) (seed 1), c = -1 for five and +1
+    for five, n in [10^5, 2*10^5) with the terms that have a prime factor below 2^16 removed."""
+    rng = np.random.default_rng(1)
+    ks = sorted(set(int(k) | 1 for k in rng.integers(3, 10**6, 10)))[:10]
+    cs = [-1] * 5 + [1] * 5
+    nmin, nmax = 100000, 199999
+    wps = (nmax - nmin + 1 + 31) // 32
+    bm = np.full(len(ks) * wps, 0xFFFFFFFF, np.uint32)
+    tail = (nmax - nmin + 1) % 32
+    if tail:
+        bm.reshape(len(ks), wps)[:, -1] &= np.uint32((1 << tail) - 1)
+    R.presieve(2, ks, cs, nmin, nmax, bm, wps, 2**16)
+    return ks, cs, nmin, nmax, bm, wps
+
+
+def _high_p_case(seed=63, K=100, nmin=10**6, nmax=2**24 - 1):
+    """b=5, 100 sequences with k in [2, 2^32) coprime to 5 (seed 63), c alternating -1, +1,
+    n in [10^6, 2^24), a bitmap keeping about 1/32 of the terms (the share a pre-sieve to 2^20
+    leaves; the AND of five random words per word)."""
+    rng = np.random.default_rng(seed)
+    ks = []
+    while len(ks) < K:
+        k = int(rng.integers(2, 2**32))
+        if k % 5 and k not in ks:
+            ks.append(k)
+    cs = [1 if i % 2 else -1 for i in range(K)]
+    wps = (nmax - nmin + 1 + 31) // 32
+    bm = rng.integers(0, 2**32, size=K * wps, dtype=np.uint32)
+    for _ in range(4):
+        bm &= rng.integers(0, 2**32, size=K * wps, dtype=np.uint32)
+    tail = (nmax - nmin + 1) % 32
+    if tail:
+        bm.reshape(K, wps)[:, -1] &= np.uint32((1 << tail) - 1)
+    return ks, cs, nmin, nmax, bm, wps
+
+
+def run_other_workloads(ctx):
+    """The other single-GPU workloads, one job each (not batched): the reference split against the
+    cost model's pick on the same terms: the small case over all of p in [10^9, 1.1*10^9), and the
+    high-p case (integer Montgomery products) on p in [2^63, 2^63 + 2^28)."""
+    from rssieve_amd import tuning
+    model = tuning.load_model(ctx)
+    res = []
+    for name, base, data, p0, p1 in (
+            ("b=2, K=10, n in [1e5,2e5) pre-sieved to 2^16, p in [1e9, 1.1e9)", 2, _small_case(), 10**9, 11 * 10**8),
+            ("b=5, K=100, n in [1e6,2^24) (1/32 of the terms), p in [2^63, 2^63+2^28)", 5, _high_p_case(),
+             2**63, 2**63 + 2**28)):
+        ks, cs, nmin, nmax, bm, wps = data
+        ref = R.reference_params(nmin, nmax, len(ks), 65536, ctx.compute_units)
+        cands = ctx.tune_candidates(nmin, nmax, len(ks))
+        x = tuning.features(base, len(ks), int(np.bitwise_count(bm).sum()), nmax - nmin + 1, np.log2(float(p0)))
+        idx = int(model.choose(x[None, :], np.array([[c.threads != 0 for c in cands]]))[0])
+        out = {}
+        for label, prm in (("plain", ref), ("tuned", cands[idx])):
+            w = ctx.job(base, ks, cs, nmin, nmax, bm.copy(), wps, prm)  # warm-up of the same shape
+            w.sieve(p0, p0 + (p1 - p0) // 64)
+            ctx.sync()
+            w.close()
+            job = ctx.job(base, ks, cs, nmin, nmax, bm.copy(), wps, prm)
+            t0 = time.perf_counter()
+            job.sieve(p0, p1)
+            ctx.sync()
+            dt = time.perf_counter() - t0
+            out[label] = (job.counters().primecount, dt, job.bitmap())
+            job.close()
+        res.append({"workload": name, "primes": int(out["plain"][0]),
+                    "plain_primes_per_s": out["plain"][0] / out["plain"][1],
+                    "tuned_primes_per_s": out["tuned"][0] / out["tuned"][1],
+                    "tuned_over_plain": out["plain"][1] / out["tuned"][1],
+                    "same_terms": bool(np.array_equal(out["plain"][2], out["tuned"][2])),
+                    "candidate": idx, "tuned_algo": int(cands[idx].algo), "tuned_Q": int(cands[idx].Q)})
+    return res
+
+
 def run_jobs16(ctx, seed=16, width=2**26):
     """Sixteen uneven jobs (bases 2, 3, 5, 6, 10; 1-100 sequences and n-spans
     10^4..10^7 log-uniform; k odd below 2^31 coprime to the base; bitmaps of
@@ -313,6 +390,8 @@ def main():
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--tuned-only", action="store_true", help="skip the plain leg (for profiling the tuned kernel)")
     ap.add_argument("--no-jobs16", action="store_true", help="skip the 16-job batch leg")
+    ap.add_argument("--other-workloads", action="store_true",
+                    help="also time the small case and the high-p case (plain vs cost-model pick)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a kernel of known traffic for PMC scaling)")
     args = ap.parse_args()
@@ -345,6 +424,7 @@ def main():
     jobs16 = None
     if not args.no_jobs16 and not args.plain_only and not args.tuned_only:
         jobs16 = run_jobs16(ctx)
+    other = run_other_workloads(ctx) if args.other_workloads else None
     ctx.close()
     value = res["primes"] / res["seconds"]
     line = {
@@ -377,6 +457,8 @@ def main():
         line["tuned_log"] = tuned["log"]
     if jobs16 is not None:
         line["jobs16"] = jobs16
+    if other is not None:
+        line["other_workloads"] = other
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(*data[:5])
     print(json.dumps(line))
