@@ -333,3 +333,36 @@ def test_high_p_production_shape_batched_and_tuned(ctx):
     for f in facs:
         i = int(f["kidx"])
         assert R.verify_factor(int(f["p"]), ks[i], int(f["n"]), cs[i], base) == 1
+
+
+@pytest.mark.gpu
+def test_small_case_whole_range_matches_oracle(ctx):
+    """The reference's CPU-sized case (b=2, ten odd k below 10^6, n in [10^5, 2*10^5)
+    pre-sieved to 2^16) over its whole p-range [10^9, 1.1*10^9), 4.81 M primes: the reference
+    split and the cost model's pick both leave the oracle's bitmap and count its primes; the
+    reference split also makes its claims."""
+    import os
+    import bench as B
+    from rssieve_amd import tuning
+    ks, cs, nmin, nmax, bm, wps = B._small_case()
+    p0, p1 = 10**9, 11 * 10**8
+    ob = bm.copy()
+    cnt, _, _ = O.sieve(O.make_job(2, ks, cs, nmin, nmax), p0, p1, ob, fac_cap=1 << 12,
+                        nthreads=max(1, min(16, os.cpu_count() or 1)))
+    assert cnt.primecount == 4814969 and cnt.claims > 0
+    model = tuning.load_model(ctx)
+    cands = ctx.tune_candidates(nmin, nmax, len(ks))
+    x = tuning.features(2, len(ks), int(np.bitwise_count(bm).sum()), nmax - nmin + 1, np.log2(float(p0)))
+    idx = int(model.choose(x[None, :], np.array([[c.threads != 0 for c in cands]]))[0])
+    for prm in (R.reference_params(nmin, nmax, len(ks), 65536, ctx.compute_units), cands[idx]):
+        print("small case, algo %d Q %d hsize %d T %d" % (prm.algo, prm.Q, prm.hsize, prm.threads), flush=True)
+        job = ctx.job(2, ks, cs, nmin, nmax, bm.copy(), wps, prm)
+        job.sieve(p0, p1)
+        ctx.sync()
+        c = job.counters()
+        got = job.bitmap()
+        job.close()
+        assert np.array_equal(got, ob)
+        assert c.primecount == cnt.primecount
+        if prm.algo == R.ALGO_THREE_LISTS:
+            assert c.claims == cnt.claims
