@@ -1129,7 +1129,10 @@ constexpr uint32_t kHitCap = 128;
 template <int T>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 : 3))) void k_bsgs(
     SegJob* __restrict__ jobs, BatchState* __restrict__ bs) {
-  constexpr int kU = 8;  // targets per giant-step group on the binary64 path at T <= 256
+  // targets per giant-step group on the binary64 path at T <= 256: six keep
+  // the kernel within 168 VGPRs without spills (eight spilled ten registers
+  // and ran 6 % slower, seven spilled one and ran 3.5 % slower)
+  constexpr int kU = 6;
   constexpr int kLgT = T >= 1024 ? 10 : T >= 512 ? 9 : T >= 256 ? 8 : 7;  // log2(T)
   static_assert((1 << kLgT) == T, "T is a power of two from 128 to 1024");
   __shared__ uint64_t s_pw[2][kLgT];  // per-item power tables (see vstart)
@@ -1511,10 +1514,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         const uint32_t qq = tid + r * T;
         uint32_t i = 0;
         if constexpr (T <= 256) {
-          // Groups of eight targets, two in flight: group g's filter words
-          // are read while group g+1's products are computed, and group
-          // g+2's targets are read meanwhile.  At T <= 256 the LDS allows two
-          // waves per SIMD, so the extra registers cost no occupancy.
+          // Groups of kU targets, two in flight: group g's products are
+          // computed while group g-1's filter words arrive, then group g-1
+          // is tested and group g+1's targets and group g's filter words are
+          // read.  At T <= 256 the LDS holds three workgroups per CU, as
+          // many as 168 VGPRs allow, so the two register sets cost no
+          // occupancy.
           auto consume = [&](const uint64_t(&uu)[kU], const uint32_t(&ww)[kU], uint32_t i0) {
             uint32_t x[kU], mn = 0xFFFFFFFFu;
 #pragma unroll
