@@ -40,19 +40,19 @@ hipError_t launch_round_bsgs(SegJob* jobs, BatchState* bs, uint32_t threads, uin
 
 using namespace rs;
 
-static thread_local std::string g_err;
+// The message of the last failure on this thread, in a fixed buffer (setting
+// it allocates nothing, so error paths of allocation-free calls stay so).
+static thread_local char g_err[512];
 
 namespace rs {
-void set_error(const std::string& msg) { g_err = msg; }
+void set_error(const std::string& msg) { snprintf(g_err, sizeof(g_err), "%s", msg.c_str()); }
 }  // namespace rs
 
 static int fail(int code, const char* fmt, ...) {
-  char buf[512];
   va_list ap;
   va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
   va_end(ap);
-  g_err = buf;
   return code;
 }
 
@@ -123,7 +123,7 @@ struct rs_job {
   uint64_t acc_launches = 0;
 };
 
-extern "C" const char* rs_last_error(void) { return g_err.c_str(); }
+extern "C" const char* rs_last_error(void) { return g_err; }
 
 extern "C" int rs_open(int device, rs_ctx** out) {
   if (!out) return fail(RS_E_ARG, "rs_open: null out");

@@ -1492,7 +1492,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       for (uint32_t i = tid; i < nk; i += T) s_q[i] = make_uint4((uint32_t)s_h[i], (uint32_t)(s_h[i] >> 32), i, 0u);
       __syncthreads();
       match(nk, (order + stride - 1) / stride, order);
-      work = order / stride + nk;
+      work = order / stride;  // the baby steps up to the repeat (giant.cl:121-134); no giant products
       probes = nk;
     } else if (fp) {
       // giant steps in binary64: the targets (Montgomery residues) times the

@@ -58,12 +58,10 @@ def draw_descriptors(n, seed, log2p_range=(32.0, 62.0)):
     return out
 
 
-def synth_job(ctx, d, params, cache=None):
-    """A job of descriptor d: seeded k coprime to the base (odd for even bases,
-    even for odd ones), c alternating, a random bitmap whose bits are set
-    with probability 2^-round(-log2 density).  A dict passed as cache keeps
-    the generated bitmaps for later calls with the same descriptor."""
-    rng = np.random.default_rng(d["seed"])
+def synth_sequences(d, rng=None):
+    """The sequences of descriptor d: K seeded k below 2^31 coprime to the
+    base (odd for even bases, even for odd ones), c alternating -1, +1."""
+    rng = np.random.default_rng(d["seed"]) if rng is None else rng
     base, K = d["base"], d["K"]
     ks = []
     while len(ks) < K:
@@ -72,6 +70,17 @@ def synth_job(ctx, d, params, cache=None):
             continue
         ks.append(k)
     cs = [1 if i % 2 else -1 for i in range(K)]
+    return ks, cs
+
+
+def synth_job(ctx, d, params, cache=None):
+    """A job of descriptor d: the sequences of synth_sequences and a random
+    bitmap whose bits are set with probability 2^-round(-log2 density).  A
+    dict passed as cache keeps the generated bitmaps for later calls with the
+    same descriptor."""
+    rng = np.random.default_rng(d["seed"])
+    base, K = d["base"], d["K"]
+    ks, cs = synth_sequences(d, rng)
     nmin = d["nmin"]
     nmax = nmin + d["span"] - 1
     wps = (nmax - nmin + 1 + 31) // 32

@@ -14,7 +14,7 @@ import pytest
 
 import oracle as O
 import rssieve_amd as R
-from util import bitmap_terms, full_bitmap, oracle_run
+from util import cleared_terms, full_bitmap, oracle_run
 
 pytestmark = pytest.mark.gpu
 
@@ -58,11 +58,14 @@ def _compare(base, ks, cs, nmin, nmax, wps, bm_in, gpu, orc, k31=False):
     assert gcnt["skipped"] == ocnt["skipped"]
     assert list(gcnt["lists"]) == [ocnt["n_full"], ocnt["n_even"], ocnt["n_odd"]]
     assert gcnt["order_lists"] == ocnt["order_lists"]
+    # Montgomery products of the reference's three tables: baby steps up to
+    # the table size (or the repeat of the order shortcut) plus M * (targets
+    # + 1) giant-step products per list
+    assert gcnt["work_mmul"] == ocnt["mmul_work"]
     assert gcnt["claims"] == ocnt["claims"]
     assert gcnt["dups"] == ocnt["dups"]
     assert np.array_equal(gbm, obm)
-    kc = len(ks)
-    elim = bitmap_terms(bm_in, kc, wps, nmin) - bitmap_terms(gbm, kc, wps, nmin)
+    elim = cleared_terms(bm_in, gbm, wps, nmin)
     gterms = {(int(f["kidx"]), int(f["n"])) for f in gfac}
     oterms = {(k, n) for (_, n, k) in ofac}
     assert gterms == oterms == elim
@@ -366,3 +369,37 @@ def test_small_case_whole_range_matches_oracle(ctx):
         assert c.primecount == cnt.primecount
         if prm.algo == R.ALGO_THREE_LISTS:
             assert c.claims == cnt.claims
+
+
+# Binary64 giant steps in whole workgroup rounds (F = M / T >= 1) below 2^51:
+# n-spans of 3e5 and 6e5 give the reference split Q = 1024 and M = 293 or 586
+# giant steps (parity lists: 147 or 293), so a 256-thread workgroup runs one or
+# two full rounds and a 512-thread one a full round, plus a partial round; a
+# one-entry queue makes every round boundary resolve (and redo) inside the
+# full rounds.  The integer path gets the same shapes above 2^51.
+FULL_ROUND_CASES = [
+    # name, base, K, nmin, nmax, pmin, pmax, seed, density, threads, queue_cap
+    ("f1_t256", 2, 8, 100000, 400000, 2**50, 2**50 + 2 * 10**5, 31, 0.5, 256, 0),
+    ("f1_t256_q1", 2, 8, 100000, 400000, 2**50, 2**50 + 2 * 10**5, 31, 0.5, 256, 1),
+    ("f2_t256", 2, 6, 100000, 700000, 2**44, 2**44 + 10**5, 32, 0.5, 256, 0),
+    ("f1_t512", 2, 8, 100000, 700000, 2**50, 2**50 + 10**5, 33, 0.5, 512, 0),
+    ("f1_t512_q1", 3, 6, 100000, 700000, 2**48, 2**48 + 10**5, 34, 0.5, 512, 1),
+    ("f1_t128", 5, 4, 100000, 400000, 2**46, 2**46 + 10**5, 35, 0.5, 128, 0),
+    ("int_f2_t256", 2, 6, 100000, 700000, 2**52, 2**52 + 10**5, 36, 0.5, 256, 0),
+]
+
+
+@pytest.mark.parametrize("case", FULL_ROUND_CASES, ids=[c[0] for c in FULL_ROUND_CASES])
+def test_full_giant_rounds_match_oracle(ctx, case):
+    name, base, K, nmin, nmax, pmin, pmax, seed, dens, threads, qcap = case
+    print("full-round case", case, flush=True)
+    ks, cs = _draw(base, K, seed)
+    bm, wps = full_bitmap(K, nmin, nmax, density=dens, seed=seed)
+    prm = R.reference_params(nmin, nmax, K, 65536, ctx.compute_units)
+    M, QQ, MM = R.derive(prm, nmin, nmax)
+    assert M >= threads  # at least one full round on the full list
+    prm.threads = threads
+    prm.queue_cap = qcap
+    gpu = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, prm)
+    orc = oracle_run(base, ks, cs, nmin, nmax, bm, pmin, pmax, nthreads=8)
+    _compare(base, ks, cs, nmin, nmax, wps, bm, gpu, orc)

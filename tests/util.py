@@ -24,6 +24,21 @@ def bitmap_terms(bm, kcount, wps, nmin):
     return set(zip(ks.tolist(), (offs + nmin).tolist()))
 
 
+def cleared_terms(bm_in, bm_out, wps, nmin):
+    """Set of (kidx, n) whose bit is set in bm_in and clear in bm_out, found
+    from the changed words only (fast for production-size bitmaps)."""
+    diff = np.asarray(bm_in, np.uint32) & ~np.asarray(bm_out, np.uint32)
+    out = set()
+    for w in np.nonzero(diff)[0].tolist():
+        v = int(diff[w])
+        kidx, off = divmod(w, wps)
+        while v:
+            b = (v & -v).bit_length() - 1
+            out.add((kidx, nmin + 32 * off + b))
+            v &= v - 1
+    return out
+
+
 def oracle_run(base, ks, cs, nmin, nmax, bm, pmin, pmax, k31=False, nthreads=1, fac_cap=1 << 22):
     job = O.make_job(base, list(ks), list(cs), nmin, nmax, k31_compat=k31)
     b = bm.copy()
