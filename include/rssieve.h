@@ -140,6 +140,21 @@ int rs_job_sequence(const rs_job *job, uint32_t kidx, uint32_t *k, int32_t *c);
  * of cl_sieve.cpp:1291-1403 (prime generation, setup, BSGS) with no host
  * round trip per segment.  Returns once enqueued. */
 int rs_sieve(rs_job *job, uint64_t pmin, uint64_t pmax);
+/* Allocation-free variant of rs_sieve, with the same rounds and results.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The BSGS overflow-recovery tables go to a static device array
+ * of RS_NOALLOC_SCRATCH_BYTES per GPU instead of context memory, and no
+ * timing events are recorded, so the call makes no hipMalloc, hipFree,
+ * hipEventCreate or other allocating call (the job's own buffers come from
+ * rs_job_create).  RS_E_ARG if the job's launch needs more than that array:
+ * its persistent BSGS workgroups times a table of the least power of two >=
+ * twice the largest baby-step list (>= 64) 32-bit words.  Every call on one
+ * GPU, from any context, shares the array and leaves its contents behind:
+ * calls on one context are stream-ordered, calls that overlap on one GPU
+ * (several streams or contexts at once) race. */
+#define RS_NOALLOC_SCRATCH_BYTES (48u << 20)
+int rs_sieve_noalloc(rs_job *job, uint64_t pmin, uint64_t pmax);
 /* Enqueue several jobs into one persistent BSGS launch per segment round
  * (no reference counterpart: the reference runs one job per process). */
 int rs_sieve_batch(rs_job **jobs, int njobs, const uint64_t *pmin, const uint64_t *pmax);

@@ -101,6 +101,7 @@ _SIGS = {
     "rs_job_view": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "rs_job_sequence": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]),
     "rs_sieve": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
+    "rs_sieve_noalloc": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
     "rs_sieve_batch": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "rs_sync": (C.c_int, [C.c_void_p]),
     "rs_counters_read": (C.c_int, [C.c_void_p, C.POINTER(Counters)]),
@@ -366,6 +367,11 @@ class Job:
 
     def sieve(self, pmin, pmax):
         _check(lib().rs_sieve(self._h, pmin, pmax))
+
+    def sieve_noalloc(self, pmin, pmax):
+        """rs_sieve_noalloc: the same sieve with no device allocation (static
+        recovery scratch, no timing events)."""
+        _check(lib().rs_sieve_noalloc(self._h, pmin, pmax))
 
     def counters(self):
         c = Counters()

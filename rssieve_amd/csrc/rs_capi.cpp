@@ -36,6 +36,9 @@ hipError_t launch_round_prep(SegJob* jobs, BatchState* bs, uint32_t njobs, uint3
                              hipStream_t s);
 hipError_t launch_round_bsgs(SegJob* jobs, BatchState* bs, uint32_t threads, uint32_t hsize, uint32_t bsgs_blocks,
                              hipStream_t s);
+size_t static_scratch_bytes();
+hipError_t static_scratch_address(int device, uint32_t** out);
+hipError_t launch_set_scratch(BatchState* bs, uint32_t* ptr, uint64_t words, hipStream_t s);
 }  // namespace rs
 
 using namespace rs;
@@ -87,6 +90,8 @@ struct rs_ctx {
   // global scratch of the BSGS overflow recovery (one table per workgroup)
   uint32_t* d_scratch = nullptr;
   size_t scratch_bytes = 0;
+  // the device's static recovery scratch of rs_sieve_noalloc (its address on this device)
+  uint32_t* static_scratch = nullptr;
   // captured rounds per batch shape
   std::vector<std::pair<GraphKey, GraphExecPair>> graphs;
 };
@@ -142,6 +147,12 @@ extern "C" int rs_open(int device, rs_ctx** out) {
   if (e != hipSuccess) {
     delete c;
     return fail(RS_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  e = static_scratch_address(device, &c->static_scratch);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(RS_E_HIP, "rs_open: address of the static scratch: %s", hipGetErrorString(e));
   }
   if (hipMalloc((void**)&c->d_bjobs, sizeof(SegJob) * RS_MAX_BATCH) != hipSuccess ||
       hipMalloc((void**)&c->d_bstate, sizeof(BatchState)) != hipSuccess) {
@@ -516,6 +527,32 @@ extern "C" int rs_sieve(rs_job* j, uint64_t pmin, uint64_t pmax) {
                         e1, s));
   }
   HIPCHK(hipEventRecord(j->ev_all[1], s));
+  return RS_OK;
+}
+
+extern "C" int rs_sieve_noalloc(rs_job* j, uint64_t pmin, uint64_t pmax) {
+  if (!j) return fail(RS_E_ARG, "rs_sieve_noalloc: null job");
+  if (pmin < 3) return fail(RS_E_ARG, "rs_sieve_noalloc: pmin %llu < 3", (unsigned long long)pmin);
+  if (pmax < pmin) return fail(RS_E_ARG, "rs_sieve_noalloc: pmax < pmin");
+  rs_ctx* ctx = j->ctx;
+  const uint64_t words = recovery_table_words(job_max_entries(j->h));
+  const size_t need = (size_t)j->bsgs_blocks * words * 4;
+  if (need > static_scratch_bytes())
+    return fail(RS_E_ARG, "rs_sieve_noalloc: %u BSGS workgroups with recovery tables of %llu words need %zu bytes, "
+                "above the %zu-byte static scratch (use rs_sieve, or a smaller table or fewer blocks per CU)",
+                j->bsgs_blocks, (unsigned long long)words, need, static_scratch_bytes());
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  uint64_t range;
+  uint32_t gen_blocks;
+  segment_shape(j, pmin, &range, &gen_blocks);
+  HIPCHK(launch_set_cursor(j->d_sj, pmin, pmax, j->h.threads, range, gen_blocks, s));
+  HIPCHK(launch_set_scratch(j->d_bs, ctx->static_scratch, words, s));
+  const uint64_t R = rounds_for(pmin, pmax, range);
+  const uint32_t setup_blocks = (j->seg_cap + 3) / 4;
+  for (uint64_t r = 0; r < R; r++)
+    HIPCHK(launch_round(j->d_sj, j->d_bs, 1, gen_blocks, setup_blocks, j->h.threads, j->h.hsize, j->bsgs_blocks,
+                        nullptr, nullptr, s));
   return RS_OK;
 }
 

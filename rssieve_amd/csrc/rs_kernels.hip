@@ -20,6 +20,9 @@
 //
 // Because the cursors live on the device, the round can be captured once in
 // a HIP graph and replayed until every job's range is done.
+#include <mutex>
+
+#include "../../include/rssieve.h"
 #include "rs_internal.h"
 #include "rs_mont.h"
 
@@ -1804,6 +1807,54 @@ hipError_t launch_round_bsgs(SegJob* jobs, BatchState* bs, uint32_t threads, uin
     case 1024: hipLaunchKernelGGL(k_bsgs<1024>, dim3(bsgs_blocks), dim3(1024), lds, s, jobs, bs); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Static scratch of the allocation-free sieve (rs_sieve_noalloc): the BSGS
+// overflow-recovery tables of every workgroup live in this file-scope device
+// array instead of context memory.  Every GPU the library runs on holds its
+// own copy from the module load until the process ends; host code uses the
+// address hipGetSymbolAddress returns for the current device (the array's
+// name on the host is only a stand-in), resolved once per device by
+// static_scratch_address at context set-up.  The recovery kernel clears the
+// part it uses before every use, so nothing in it needs to start at zero.
+// All calls on one GPU share it.
+// ---------------------------------------------------------------------------
+constexpr size_t kStaticScratchBytes = RS_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) uint32_t scratch[kStaticScratchBytes / 4];
+
+size_t static_scratch_bytes() { return kStaticScratchBytes; }
+
+hipError_t static_scratch_address(int device, uint32_t** out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static uint32_t* addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {  // first context on this device: the caller made it current
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    addr[device] = (uint32_t*)p;
+  }
+  *out = addr[device];
+  return hipSuccess;
+}
+
+// Points a launch's work queue at a recovery scratch of 'words' 32-bit words
+// per workgroup (a kernel rather than a copy, so the call records no host
+// buffer and can be captured).
+__global__ void k_set_scratch(BatchState* __restrict__ bs, uint32_t* ptr, uint64_t words) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    bs->scratch = ptr;
+    bs->scratch_words = words;
+  }
+}
+
+hipError_t launch_set_scratch(BatchState* bs, uint32_t* ptr, uint64_t words, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_scratch, dim3(1), dim3(64), 0, s, bs, ptr, words);
   return hipGetLastError();
 }
 
