@@ -21,6 +21,7 @@
 // Because the cursors live on the device, the round can be captured once in
 // a HIP graph and replayed until every job's range is done.
 #include <mutex>
+#include <type_traits>
 
 #include "../../include/rssieve.h"
 #include "rs_internal.h"
@@ -1042,32 +1043,70 @@ __device__ __noinline__ int gtab_find_at(const uint32_t* tab, uint32_t bmask, ui
 }
 
 // Blocked Bloom filter of the baby steps: hsize 32-bit words in LDS, four
-// bits per baby value, all in the word chosen by the residue's low bits.
-// With hsize/2 values the false-positive rate is about 4.7e-4 (three bits
-// gave 9.9e-4, two 2.8e-3), with hsize values about 1.9e-3 (three bits:
-// 3e-3), so a giant step costs one ds_read_b32 and a few integer ops; the
-// rare positives are queued and resolved exactly, and the fewer there are,
-// the less often a wavefront takes the queueing branch (whose returning LDS
-// atomic waits for every LDS read in flight).  Residues of products mod p are
-// spread evenly over their low bits: bits 0..14 pick the word (hsize <=
-// 32768), bits 16..20, 24..28, 32..36 and 40..44 the four bits (for p below
-// 2^45 the last fields are partly zero and their bits coincide, which only
-// costs selectivity).
-__device__ __forceinline__ uint32_t flt_bits(uint64_t v) {
-  const uint32_t x = (uint32_t)v, y = (uint32_t)(v >> 32);
-  return (1u << ((x >> 16) & 31u)) | (1u << ((x >> 24) & 31u)) | (1u << (y & 31u)) | (1u << ((y >> 8) & 31u));
+// bits per baby value, one in each byte of the word its residue picks.  The
+// word is residue bits 2 .. 1 + log2(hsize), used directly as the byte offset
+// v & ((hsize - 1) << 2) (one AND); the bits are the low three bits of each
+// byte of the 32 residue bits from bit s on (one v_alignbit and one AND),
+// each made a one-hot byte by one v_perm_b32 from the table {1, 2, ..., 128}.
+// With hsize/2 values the false-positive rate is about 5e-4, with hsize
+// values about 2e-3 (simulated), so a giant step costs one ds_read_b32 and
+// four integer instructions; the rare positives are queued and resolved
+// exactly, and the fewer there are, the less often a wavefront takes the
+// queueing branch (whose returning LDS atomic waits for every LDS read in
+// flight).  s = 17 for p >= 2^45; below, the fields move down so that they
+// stay inside the residue's bits, but never into the word index (for small p
+// they then cover fewer residue bits, which only costs selectivity).
+struct Filter {
+  uint32_t* w;     // hsize words
+  uint32_t bmask;  // (hsize - 1) << 2
+  uint32_t s;      // first bit of the four 3-bit fields
+};
+__device__ __forceinline__ Filter filter_make(uint32_t* w, uint32_t hsize, uint64_t p) {
+  const uint32_t lgh = 31u - (uint32_t)__clz((int)hsize);
+  const uint32_t plen = 64u - (uint32_t)__clzll((long long)p);
+  Filter f;
+  f.w = w;
+  f.bmask = (hsize - 1u) << 2;
+  f.s = max(2u + lgh, min(17u, plen > 28u ? plen - 28u : 0u));
+  return f;
 }
-__device__ __forceinline__ void flt_insert(uint32_t* flt, uint32_t fmask, uint64_t v) {
-  atomicOr(&flt[(uint32_t)v & fmask], flt_bits(v));
+__device__ __forceinline__ uint32_t flt_bits(const Filter& f, uint64_t v) {
+  const uint32_t sel = __builtin_amdgcn_alignbit((uint32_t)(v >> 32), (uint32_t)v, f.s) & 0x07070707u;
+  return __builtin_amdgcn_perm(0x80402010u, 0x08040201u, sel);
 }
-__device__ __forceinline__ bool flt_maybe(const uint32_t* flt, uint32_t fmask, uint64_t v) {
-  const uint32_t m = flt_bits(v);
-  return (flt[(uint32_t)v & fmask] & m) == m;
+__device__ __forceinline__ uint32_t* flt_word(const Filter& f, uint64_t v) {
+  return (uint32_t*)((char*)f.w + ((uint32_t)v & f.bmask));
 }
-// the filter bits of v that are clear: 0 iff v may be in the table
-__device__ __forceinline__ uint32_t flt_miss(const uint32_t* flt, uint32_t fmask, uint64_t v) {
-  return ~flt[(uint32_t)v & fmask] & flt_bits(v);
-}
+__device__ __forceinline__ void flt_insert(const Filter& f, uint64_t v) { atomicOr(flt_word(f, v), flt_bits(f, v)); }
+
+// The giant-step products of the tile loop in two arithmetics: binary64 plain
+// residues for odd p < 2^51 (a probe's key is kFpBias + the residue, the
+// integer residue is the key without the bias), integer Montgomery residues
+// otherwise.  Targets are Montgomery residues and the giant powers plain
+// residues (binary64) or Montgomery residues (integer), so both give the
+// Montgomery residue the table holds.
+struct GiantFp {
+  double pd, pinv, pc;
+  using Tgt = double;
+  using Pow = double;
+  __device__ Tgt target(uint64_t bits) const { return __longlong_as_double((long long)bits); }
+  __device__ Pow power(uint64_t bits) const { return __longlong_as_double((long long)bits); }
+  __device__ Pow unit() const { return 1.0; }
+  __device__ Pow mul(Pow a, Pow b) const { return fp_mulmod(a, b, pd, pinv); }
+  __device__ uint64_t key(Tgt t, Pow g) const { return fp_mulmod_bits(t, g, pd, pinv, pc); }
+  __device__ uint64_t value(uint64_t key) const { return key ^ kFpBias; }
+};
+struct GiantInt {
+  uint64_t p, q, one;
+  using Tgt = uint64_t;
+  using Pow = uint64_t;
+  __device__ Tgt target(uint64_t bits) const { return bits; }
+  __device__ Pow power(uint64_t bits) const { return bits; }
+  __device__ Pow unit() const { return one; }
+  __device__ Pow mul(Pow a, Pow b) const { return mmul(a, b, p, q); }
+  __device__ uint64_t key(Tgt t, Pow g) const { return mmul(t, g, p, q); }
+  __device__ uint64_t value(uint64_t key) const { return key; }
+};
 
 // Hash of the match pass's filter over the queued positives.  It must not
 // reuse the main filter's fields: a false positive passed the main filter
@@ -1132,10 +1171,10 @@ constexpr uint32_t kHitCap = 128;
 template <int T>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 : 3))) void k_bsgs(
     SegJob* __restrict__ jobs, BatchState* __restrict__ bs) {
-  // targets per giant-step group on the binary64 path at T <= 256: six keep
-  // the kernel within 168 VGPRs without spills (eight spilled ten registers
-  // and ran 6 % slower, seven spilled one and ran 3.5 % slower)
-  constexpr int kU = 6;
+  // targets per giant-step tile: six at T <= 256 keep the kernel within 168
+  // VGPRs (eight spilled and ran 6 % slower), four at T >= 512 (128 VGPRs)
+  constexpr uint32_t kU = T <= 256 ? 6 : 4;
+  constexpr uint32_t kW = T / 64;  // wavefronts per workgroup
   constexpr int kLgT = T >= 1024 ? 10 : T >= 512 ? 9 : T >= 256 ? 8 : 7;  // log2(T)
   static_assert((1 << kLgT) == T, "T is a power of two from 128 to 1024");
   __shared__ uint64_t s_pw[2][kLgT];  // per-item power tables (see vstart)
@@ -1205,7 +1244,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       s_qn = 0;
     }
     __syncthreads();
-    const uint32_t item = s_item, jj = s_job;
+    // workgroup-uniform values read from LDS go through readfirstlane, so
+    // the compiler keeps the item's values (and the loads that follow from
+    // them) in scalar registers
+    const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_item), jj = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_job);
     __syncthreads();
     if (item >= nitems) break;
     const SegJob& J = jobs[jj];
@@ -1214,16 +1256,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     const uint32_t local = item - bs->item_base[jj];
     const uint32_t pi = merged ? local : local / 3, c = merged ? 0 : local % 3;
     const PrimeRec* __restrict__ R = J.recs + pi;
-    const uint32_t nk = R->nk[c];
+    const uint32_t nk = (uint32_t)__builtin_amdgcn_readfirstlane((int)R->nk[c]);
     if (nk == 0) continue;
-    const uint32_t hsize = job->hsize, fmask = hsize - 1;
-    uint32_t* flt = tab;
-    uint64_t* s_g = (uint64_t*)(tab + hsize);  // T giant-step powers, after the filter
+    const uint32_t hsize = job->hsize;
     const uint32_t base = job->base;
     const int K = job->kcount;
     const uint32_t nmin = job->nmin, nmax = job->nmax;
     const uint32_t cp = (c && !merged) ? 1 : 0;
     const uint64_t p = R->p, q = R->q, one = R->one, mbase = R->mbase;
+    const Filter flt = filter_make(tab, hsize, p);
     // table stride, baby-step entries, exponent span and giant steps of the
     // item: the reference's (three lists), fixed (merged), or per prime
     // (residue classes)
@@ -1231,7 +1272,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     const uint32_t stride = rcls ? R->S : (cp || merged) ? 2 : 1;
     const uint32_t nent = rcls ? R->E : (cp ? job->QQ : job->Q) / stride;
     const uint32_t Ql = rcls ? stride * nent : cp ? job->QQ : job->Q;
-    const uint32_t Ml = rcls ? R->M : cp ? job->MM : job->M;
+    const uint32_t Ml = (uint32_t)__builtin_amdgcn_readfirstlane((int)(rcls ? R->M : cp ? job->MM : job->M));
     const uint32_t nstart = nmin + (c == 2 ? 1 : 0);
     const uint64_t v0 = (c == 2) ? mul_base(R->bn, mbase, base, p, q) : R->bn;
     const uint64_t bstep = R->bT[cp];
@@ -1248,14 +1289,23 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       const uint32_t slot = atomicAdd(&s_qn, 1u);
       if (slot < qcap) s_q[slot] = make_uint4((uint32_t)g, (uint32_t)(g >> 32), i, qi);
     };
-    // Giant steps run in F = Ml / T full rounds (q = tid + r*T) and a last
-    // partial round of m = Ml - F*T steps spread over all threads.
-    const uint32_t F = Ml / T, m = Ml - F * T;
-    // Rounds between queue checks: a round queues about T * nk * 4.7e-4
+    // Giant steps in tiles: a tile is 64 consecutive giant steps, one per
+    // lane (q = 64*qb + lane), times kU consecutive targets (group g); tile
+    // t = qb*ng + g, and wavefront w of the workgroup takes the tiles w,
+    // w + kW, w + 2 kW, ... (step j holds the tiles j*kW .. j*kW + kW - 1).
+    // Every probe of the item, whatever M and the target count, then runs
+    // through the one software-pipelined loop below, and the wavefronts get
+    // equal shares (lanes with q >= M and targets past nk are computed and
+    // ignored).
+    // (the wavefront index through readfirstlane: the tile walk is scalar)
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6)), lane = tid & 63;
+    const uint32_t ng = (nk + kU - 1) / kU, nblk = (Ml + 63) / 64, ntiles = nblk * ng;
+    const uint32_t nsteps = (ntiles + kW - 1) / kW;
+    // Steps between queue checks: a step queues about T * kU * 5e-4
     // positives (the filter's rate at half load); checking when a quarter of
-    // the queue could be filled at 2e-3 (one value per word) keeps overflows rare without a
-    // barrier per round.
-    const uint32_t every = max(1u, 125u * qcap / (T * nk));
+    // the queue could be filled at 2e-3 (one value per word) keeps overflows
+    // rare without a barrier per step.
+    const uint32_t every = max(1u, 125u * qcap / (T * kU));
 
     // Baby steps: entry e = tid + T*i is b^(nstart + stride*e).  Below 2^51
     // the steps multiply in binary64 by the plain residue of the stride,
@@ -1362,7 +1412,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         });
       });
       __syncthreads();
-      const uint32_t nh = s_nhit;
+      const uint32_t nh = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_nhit);
       if (nh <= kHitCap) {
         for (uint32_t s = tid; s < nh; s += T) claim_hit(s_hit[s].x, s_hit[s].y, ord);
       } else {
@@ -1376,9 +1426,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
 
     // The overflow recovery: the bucketed table of the baby values is built
     // in this workgroup's global scratch (at most half full) and every probe
-    // of rounds [ra, F) and of the last partial round is looked up directly
-    // with integer Montgomery products h * G^q.
-    auto redo = [&](uint32_t ra) {
+    // of the tiles of steps ja on is looked up directly with integer
+    // Montgomery products h * G^q.
+    auto redo = [&](uint32_t ja) {
       const uint32_t tw = recovery_table_words(nent), gbmask = tw / kBucket - 1;
       uint32_t* gtab = bs->scratch + (size_t)blockIdx.x * bs->scratch_words;
       __syncthreads();
@@ -1403,23 +1453,20 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         }
       };
       const uint64_t G = R->g[cp];
-      for (uint32_t rr = ra; rr < F; rr++) {
-        const uint32_t qi = tid + rr * T;
+      for (uint32_t t = ja * kW + wave; t < ntiles; t += kW) {
+        const uint32_t qi = (t / ng) * 64 + lane, g0 = (t % ng) * kU;
+        if (qi >= Ml) continue;
         const uint64_t Gq = powmod_sm(G, qi, p, q, one);
-        for (uint32_t i = 0; i < nk; i++) resolve(mmul(kr[i].hadj, Gq, p, q), i, qi);
-      }
-      for (uint32_t L = tid; L < m * nk; L += T) {
-        const uint32_t qi = F * T + L % m, i = L / m;
-        resolve(mmul(kr[i].hadj, powmod_sm(G, qi, p, q, one), p, q), i, qi);
+        for (uint32_t i = g0; i < g0 + kU && i < nk; i++) resolve(mmul(kr[i].hadj, Gq, p, q), i, qi);
       }
     };
-    // After the rounds before r1 (block-uniform): resolve the queue when it
-    // is half full or at the end.  If it overflowed, the positives of rounds
-    // [r0, r1) are incomplete: redo every round from r0 on directly and
+    // After the steps before r1 (block-uniform): resolve the queue when it
+    // is half full or at the end.  If it overflowed, the positives of steps
+    // [r0, r1) are incomplete: redo every step from r0 on directly and
     // report the item finished (true).
     auto settle = [&](uint32_t& r0, uint32_t r1, bool final) -> bool {
       __syncthreads();
-      const uint32_t nq = s_qn;
+      const uint32_t nq = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_qn);
       if (nq > qcap) {
         redo(r0);
         return true;
@@ -1463,7 +1510,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     // item, so nothing waits for these loads on the way.
     uint32_t pf = 0;
     {
-      const uint32_t nx = s_next;
+      const uint32_t nx = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_next);
       // the targets of a prime rarely exceed K (room is 2K): K records' lines
       const uint32_t kbytes = (uint32_t)K * (uint32_t)sizeof(KRec);
       if (nx != 0xFFFFFFFFu && (tid == 0 || 128u * (tid - 1) < kbytes)) {
@@ -1481,11 +1528,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       baby_walk(nent, [&](uint32_t e, uint64_t v) {
         const uint32_t j = e * stride;
         if (j && v == v0m) atomicMin(&s_order, j);
-        if ((uint32_t)v) flt_insert(flt, fmask, v);
+        if ((uint32_t)v) flt_insert(flt, v);
       });
     }
     __syncthreads();
-    const uint32_t order = s_order;
+    const uint32_t order = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_order);
     unsigned long long work, probes;
     if (order != 0xFFFFFFFFu) {
       // ord_p(b) (or its even multiple) is below the table span: the baby
@@ -1497,222 +1544,119 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       match(nk, (order + stride - 1) / stride, order);
       work = order / stride;  // the baby steps up to the repeat (giant.cl:121-134); no giant products
       probes = nk;
-    } else if (fp) {
-      // giant steps in binary64: the targets (Montgomery residues) times the
-      // plain residue of G^q give the Montgomery residues the table holds;
-      // fp_mulmod_bits leaves kFpBias + residue (same low 51 bits).  Thread
-      // tid takes q = tid + r*T in F full rounds, four targets at a time so
-      // four products and four filter reads are in flight together.
-      const double pc = pd + kFpTwo52;
-      double gq = 1.0;  // G^tid, plain
+    } else {
+      // The wavefront's tiles, software-pipelined: tile j's products are
+      // computed while tile j-1's filter words arrive, then tile j-1 is
+      // tested and the next tile's targets and tile j's filter words are
+      // read.  Two register sets alternate (no copies, which would wait for
+      // the loads).  A tile is consumed before the next reads are issued, so
+      // a positive's returning LDS atomic waits only for reads issued half an
+      // iteration earlier.
+      auto giant = [&](const auto& ar) __attribute__((always_inline)) {
+        using A = typename std::decay<decltype(ar)>::type;
+        using Tg = typename A::Tgt;
+        using Pw = typename A::Pow;
+        // this wavefront's first tile and its giant power G^(64*qb + lane)
+        // (64*qb + lane < T: popcount products from the power table)
+        uint32_t qb = wave / ng, g = wave % ng;
+        Pw gq = ar.unit();
+        {
+          const uint32_t e0 = 64 * qb + lane;
 #pragma unroll
-      for (int i = 0; i < kLgT; i++)
-        if ((tid >> i) & 1u) gq = fp_mulmod(gq, __longlong_as_double(s_pw[1][i]), pd, pinv);
-      const double gtd = (double)mmul(R->gT[cp], 1, p, q);
-      for (uint32_t i = tid; i < nk; i += T) s_h[i] = (uint64_t)__double_as_longlong((double)s_h[i]);
-      __syncthreads();
-      uint32_t r0 = 0;
-      bool done = false;  // the overflow recovery finished the item
-      for (uint32_t r = 0; r < F; r++) {
-        const uint32_t qq = tid + r * T;
-        uint32_t i = 0;
-        if constexpr (T <= 256) {
-          // Groups of kU targets, two in flight: group g's products are
-          // computed while group g-1's filter words arrive, then group g-1
-          // is tested and group g+1's targets and group g's filter words are
-          // read.  At T <= 256 the LDS holds three workgroups per CU, as
-          // many as 168 VGPRs allow, so the two register sets cost no
-          // occupancy.
-          auto consume = [&](const uint64_t(&uu)[kU], const uint32_t(&ww)[kU], uint32_t i0) {
-            uint32_t x[kU], mn = 0xFFFFFFFFu;
+          for (int i = 0; i < kLgT; i++)
+            if ((e0 >> i) & 1u) gq = ar.mul(gq, ar.power(s_pw[1][i]));
+        }
+        const Pw g64 = ar.power(s_pw[1][6]);  // G^64: one tile row further
+        Tg tt[kU];
+        auto load_t = [&]() {
 #pragma unroll
-            for (int k = 0; k < kU; k++) {
-              x[k] = ~ww[k] & flt_bits(uu[k]);
-              mn = min(mn, x[k]);
-            }
-            if (mn == 0u) {
+          for (uint32_t k = 0; k < kU; k++) tt[k] = ar.target(s_h[g * kU + k]);
+        };
+        // products of the current tile (targets in tt), then advance to the
+        // wavefront's next tile
+        auto mul_t = [&](uint64_t(&uu)[kU], uint32_t& i0, uint32_t& qbt) {
 #pragma unroll
-              for (int k = 0; k < kU; k++)
-                if (!x[k] && i0 + k < nk) defer(uu[k] ^ kFpBias, i0 + k, qq);
-            }
-          };
-          // Ping-pong between two register sets (no copies between
-          // iterations, which would wait for the loads); target reads run
-          // one group ahead without a bounds test (s_h is padded).  The last
-          // group is padded too: its probes past nk are computed and ignored,
-          // so every probe goes through the pipeline.
-          const uint32_t ng = (nk + kU - 1) / kU;
-          if (ng) {
-            double t[kU];
+          for (uint32_t k = 0; k < kU; k++) uu[k] = ar.key(tt[k], gq);
+          i0 = g * kU;
+          qbt = qb;
+          g = (uint32_t)__builtin_amdgcn_readfirstlane((int)(g + kW));
+          while (g >= ng) {
+            g -= ng;
+            qb++;
+            gq = ar.mul(gq, g64);
+          }
+        };
+        auto read_w = [&](const uint64_t(&uu)[kU], uint32_t(&ww)[kU]) {
+#pragma unroll
+          for (uint32_t k = 0; k < kU; k++) ww[k] = *flt_word(flt, uu[k]);
+        };
+        auto consume = [&](const uint64_t(&uu)[kU], const uint32_t(&ww)[kU], uint32_t i0, uint32_t qbt) {
+          uint32_t x[kU], mn = 0xFFFFFFFFu;
+#pragma unroll
+          for (uint32_t k = 0; k < kU; k++) {
+            x[k] = ~ww[k] & flt_bits(flt, uu[k]);
+            mn = min(mn, x[k]);
+          }
+          const uint32_t qq = qbt * 64 + lane;
+          if (mn == 0u && qq < Ml) {
+#pragma unroll
+            for (uint32_t k = 0; k < kU; k++)
+              if (!x[k] && i0 + k < nk) defer(ar.value(uu[k]), i0 + k, qq);
+          }
+        };
+        load_t();
+        uint32_t r0 = 0;  // first step whose positives are not yet resolved
+        for (uint32_t j0 = 0; j0 < nsteps; j0 += every) {
+          const uint32_t j1 = min(nsteps, j0 + every);
+          // this wavefront's tiles in steps [j0, j1)
+          const uint32_t tf = j0 * kW + wave;
+          const uint32_t nt = tf < ntiles ? min(j1 - j0, (ntiles - tf + kW - 1) / kW) : 0u;
+          if (nt) {
             uint64_t uA[kU], uB[kU];
-            uint32_t wA[kU], wB[kU];
-            auto load_t = [&](uint32_t grp) {
-#pragma unroll
-              for (int k = 0; k < kU; k++) t[k] = __longlong_as_double(s_h[grp * kU + k]);
-            };
-            auto mul_t = [&](uint64_t(&uu)[kU]) {
-#pragma unroll
-              for (int k = 0; k < kU; k++) uu[k] = fp_mulmod_bits(t[k], gq, pd, pinv, pc);
-            };
-            auto read_w = [&](const uint64_t(&uu)[kU], uint32_t(&ww)[kU]) {
-#pragma unroll
-              for (int k = 0; k < kU; k++) ww[k] = flt[(uint32_t)uu[k] & fmask];
-            };
-            load_t(0);
-            mul_t(uA);
-            load_t(1);
+            uint32_t wA[kU], wB[kU], iA, iB, qA, qB;
+            mul_t(uA, iA, qA);
+            load_t();
             read_w(uA, wA);
-            uint32_t g = 1;
-            // A group is consumed before the next reads are issued: a
-            // positive's returning LDS atomic then waits only for reads
-            // issued half an iteration earlier (1 % faster than after).
-            for (; g + 1 < ng; g += 2) {  // groups g (B) and g + 1 (A)
-              mul_t(uB);
-              consume(uA, wA, (g - 1) * kU);
-              load_t(g + 1);
+            uint32_t n = 1;
+            for (; n + 1 < nt; n += 2) {
+              mul_t(uB, iB, qB);
+              consume(uA, wA, iA, qA);
+              load_t();
               read_w(uB, wB);
-              mul_t(uA);
-              consume(uB, wB, g * kU);
-              load_t(g + 2);
+              mul_t(uA, iA, qA);
+              consume(uB, wB, iB, qB);
+              load_t();
               read_w(uA, wA);
             }
-            if (g < ng) {
-              mul_t(uB);
-              consume(uA, wA, (g - 1) * kU);
+            if (n < nt) {
+              mul_t(uB, iB, qB);
+              consume(uA, wA, iA, qA);
+              load_t();
               read_w(uB, wB);
-              consume(uB, wB, g * kU);
+              consume(uB, wB, iB, qB);
             } else {
-              consume(uA, wA, (g - 1) * kU);
-            }
-            i = ng * kU;
-          }
-        } else {
-          // four targets per group, within 128 registers (four waves per SIMD)
-          for (; i + 4 <= nk; i += 4) {
-            const uint64_t u0 = fp_mulmod_bits(__longlong_as_double(s_h[i]), gq, pd, pinv, pc);
-            const uint64_t u1 = fp_mulmod_bits(__longlong_as_double(s_h[i + 1]), gq, pd, pinv, pc);
-            const uint64_t u2 = fp_mulmod_bits(__longlong_as_double(s_h[i + 2]), gq, pd, pinv, pc);
-            const uint64_t u3 = fp_mulmod_bits(__longlong_as_double(s_h[i + 3]), gq, pd, pinv, pc);
-            const uint32_t x0 = flt_miss(flt, fmask, u0), x1 = flt_miss(flt, fmask, u1);
-            const uint32_t x2 = flt_miss(flt, fmask, u2), x3 = flt_miss(flt, fmask, u3);
-            if (min(min(x0, x1), min(x2, x3)) == 0u) {
-              if (!x0) defer(u0 ^ kFpBias, i, qq);
-              if (!x1) defer(u1 ^ kFpBias, i + 1, qq);
-              if (!x2) defer(u2 ^ kFpBias, i + 2, qq);
-              if (!x3) defer(u3 ^ kFpBias, i + 3, qq);
+              consume(uA, wA, iA, qA);
             }
           }
+          if (settle(r0, j1, j1 == nsteps)) break;  // the overflow recovery finished the item
         }
-        for (; i < nk; i++) {
-          const uint64_t u0 = fp_mulmod_bits(__longlong_as_double(s_h[i]), gq, pd, pinv, pc);
-          if (flt_maybe(flt, fmask, u0)) defer(u0 ^ kFpBias, i, qq);
-        }
-        gq = fp_mulmod(gq, gtd, pd, pinv);
-        if ((r + 1) % every == 0 && settle(r0, r + 1, false)) {
-          done = true;
-          break;
-        }
-      }
-      // The last m = Ml - F*T giant steps: their m*nk probes are spread over
-      // every thread, (giant step j, target i) for L = tid, tid + T, ...
-      // with j = L mod m and i = L / m; thread j < m holds G^(F*T + j).
-      if (!done && m) {
-        if (tid < m) s_g[tid] = (uint64_t)__double_as_longlong(gq);
+      };
+      if (fp) {
+        // binary64 targets: the Montgomery residues as doubles (exact below 2^51)
+        for (uint32_t i = tid; i < nk; i += T) s_h[i] = (uint64_t)__double_as_longlong((double)s_h[i]);
         __syncthreads();
-        // four probes L, L+T, L+2T, L+3T per step, their (j, i) advanced by
-        // 4T; indices past the end are clamped into range and the probes
-        // ignored, so the loop body has no divergent loads
-        const uint32_t total = m * nk, da = (4 * T) / m, db = (4 * T) % m;
-        uint32_t jj4[4], ii4[4];
-        jj4[0] = tid % m;
-        ii4[0] = tid / m;
-#pragma unroll
-        for (int c = 1; c < 4; c++) {
-          jj4[c] = jj4[c - 1] + T % m;
-          ii4[c] = ii4[c - 1] + T / m;
-          if (jj4[c] >= m) {
-            jj4[c] -= m;
-            ii4[c]++;
-          }
-        }
-        for (uint32_t L = tid; L < total; L += 4 * T) {
-          uint64_t u[4];
-          uint32_t w[4];
-#pragma unroll
-          for (int c = 0; c < 4; c++) {
-            const uint32_t ic = min(ii4[c], nk - 1), jc = min(jj4[c], m - 1);
-            u[c] = fp_mulmod_bits(__longlong_as_double(s_h[ic]), __longlong_as_double(s_g[jc]), pd, pinv, pc);
-          }
-#pragma unroll
-          for (int c = 0; c < 4; c++) w[c] = flt[(uint32_t)u[c] & fmask];
-#pragma unroll
-          for (int c = 0; c < 4; c++) {
-            if (L + c * T < total && (~w[c] & flt_bits(u[c])) == 0u) defer(u[c] ^ kFpBias, ii4[c], F * T + jj4[c]);
-            ii4[c] += da;
-            jj4[c] += db;
-            if (jj4[c] >= m) {
-              jj4[c] -= m;
-              ii4[c]++;
-            }
-          }
-        }
+        GiantFp ar;
+        ar.pd = pd;
+        ar.pinv = pinv;
+        ar.pc = pd + kFpTwo52;
+        giant(ar);
+      } else {
+        GiantInt ar;
+        ar.p = p;
+        ar.q = q;
+        ar.one = one;
+        giant(ar);
       }
-      if (!done) settle(r0, F, true);
-      work = (unsigned long long)nent + (unsigned long long)Ml * (nk + 1);
-      probes = (unsigned long long)Ml * nk;
-    } else {
-      // giant steps with integer Montgomery products, same schedule
-      const uint64_t g = R->g[cp], gT = R->gT[cp];
-      uint64_t Gq = one;  // G^tid
-#pragma unroll
-      for (int i = 0; i < kLgT; i++)
-        if ((tid >> i) & 1u) Gq = mmul(Gq, s_pw[1][i], p, q);
-      uint32_t r0 = 0;
-      bool done = false;  // the overflow recovery finished the item
-      for (uint32_t r = 0; r < F; r++) {
-        const uint32_t qq = tid + r * T;
-        uint32_t i = 0;
-        for (; i + 4 <= nk; i += 4) {
-          const uint64_t g0 = mmul(s_h[i], Gq, p, q);
-          const uint64_t g1 = mmul(s_h[i + 1], Gq, p, q);
-          const uint64_t g2 = mmul(s_h[i + 2], Gq, p, q);
-          const uint64_t g3 = mmul(s_h[i + 3], Gq, p, q);
-          const uint32_t x0 = flt_miss(flt, fmask, g0), x1 = flt_miss(flt, fmask, g1);
-          const uint32_t x2 = flt_miss(flt, fmask, g2), x3 = flt_miss(flt, fmask, g3);
-          if (min(min(x0, x1), min(x2, x3)) == 0u) {
-            if (!x0) defer(g0, i, qq);
-            if (!x1) defer(g1, i + 1, qq);
-            if (!x2) defer(g2, i + 2, qq);
-            if (!x3) defer(g3, i + 3, qq);
-          }
-        }
-        for (; i < nk; i++) {
-          const uint64_t g0 = mmul(s_h[i], Gq, p, q);
-          if (flt_maybe(flt, fmask, g0)) defer(g0, i, qq);
-        }
-        Gq = mmul(Gq, gT, p, q);
-        if ((r + 1) % every == 0 && settle(r0, r + 1, false)) {
-          done = true;
-          break;
-        }
-      }
-      if (!done && m) {
-        if (tid < m) s_g[tid] = Gq;
-        __syncthreads();
-        const uint32_t da = T / m, db = T % m;
-        uint32_t j = tid % m, i = tid / m;
-        for (uint32_t L = tid; L < m * nk; L += T) {
-          const uint64_t g0 = mmul(s_h[i], s_g[j], p, q);
-          if (flt_maybe(flt, fmask, g0)) defer(g0, i, F * T + j);
-          i += da;
-          j += db;
-          if (j >= m) {
-            j -= m;
-            i++;
-          }
-        }
-      }
-      if (!done) settle(r0, F, true);
       work = (unsigned long long)nent + (unsigned long long)Ml * (nk + 1);
       probes = (unsigned long long)Ml * nk;
     }
@@ -1750,8 +1694,8 @@ __global__ void k_round_items(SegJob* __restrict__ jobs, BatchState* __restrict_
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-// filter (hsize u32) + one u64 power per thread
-size_t bsgs_lds_bytes(uint32_t hsize, uint32_t threads) { return (size_t)hsize * 4 + (size_t)threads * 8; }
+// dynamic LDS of a BSGS workgroup: the filter (hsize u32)
+size_t bsgs_lds_bytes(uint32_t hsize, uint32_t) { return (size_t)hsize * 4; }
 
 // static LDS of the BSGS kernel (target lists, positive queue, counters)
 hipError_t bsgs_static_lds(uint32_t threads, size_t* out) {
