@@ -252,22 +252,23 @@ __device__ __forceinline__ void powmod_lg2(uint64_t x0, uint64_t x1, uint64_t ex
   *r1 = a1;
 }
 
-// powmod_lg2 on plain residues in binary64
+// powmod_lg2 on plain residues in binary64 (signed intermediate products,
+// canonical results)
 __device__ __forceinline__ void fp_powmod_lg2(double x0, double x1, uint64_t exp, double p, double pinv, double* r0,
                                               double* r1) {
   uint64_t bit = 0x8000000000000000ull >> ((clz64(exp) + 1) & 63);
   double a0 = x0, a1 = x1;
   while (bit) {
-    a0 = fp_mulmod(a0, a0, p, pinv);
-    a1 = fp_mulmod(a1, a1, p, pinv);
+    a0 = fp_mulmod_signed(a0, a0, p, pinv);
+    a1 = fp_mulmod_signed(a1, a1, p, pinv);
     if (exp & bit) {
-      a0 = fp_mulmod(a0, x0, p, pinv);
-      a1 = fp_mulmod(a1, x1, p, pinv);
+      a0 = fp_mulmod_signed(a0, x0, p, pinv);
+      a1 = fp_mulmod_signed(a1, x1, p, pinv);
     }
     bit >>= 1;
   }
-  *r0 = a0;
-  *r1 = a1;
+  *r0 = fp_canon(a0, p);
+  *r1 = fp_canon(a1, p);
 }
 
 // d of the power-residue tests (setup.cl:22-48)
@@ -294,20 +295,38 @@ constexpr uint32_t kZH = 640;                 // their hash slots: [0, 256), [25
 constexpr uint64_t kClassMinP = 1u << 20;     // smaller p keep the merged table's parity split; above it
                                               // the exponent (p-1)/L is at least 2 (L <= 720720)
 
-// a^-1 mod m for gcd(a, m) = 1 and 1 <= m < 2^31 (extended Euclid); 0 for m = 1
-__device__ __forceinline__ uint32_t inv_mod_small(uint32_t a, uint32_t m) {
-  int r0 = (int)m, r1 = (int)(a % m), t0 = 0, t1 = 1;
-  while (r1) {
-    const int qq = r0 / r1;
-    int t = r0 - qq * r1;
-    r0 = r1;
-    r1 = t;
-    t = t0 - qq * t1;
-    t0 = t1;
-    t1 = t;
+// x mod m for x < 2^22 and 1 <= m < 2^22 without an integer division: the
+// quotient from a float reciprocal (relative error below 2^-22, so off by at
+// most one for quotients below 2^22) and one correction of the remainder
+struct SmallMod {
+  uint32_t m;
+  float inv;
+  __device__ explicit SmallMod(uint32_t mm) : m(mm), inv(__builtin_amdgcn_rcpf((float)mm)) {}
+  __device__ uint32_t operator()(uint32_t x) const {
+    const uint32_t qe = (uint32_t)((float)x * inv);
+    int r = (int)x - (int)__umul24(qe, m);
+    if (r < 0) r += (int)m;
+    else if (r >= (int)m) r -= (int)m;
+    return (uint32_t)r;
   }
-  return t0 < 0 ? (uint32_t)(t0 + (int)m) : (uint32_t)t0;
+};
+
+// a^-1 mod m for gcd(a, m) = 1 and m <= 192, by every lane of the wavefront
+// trying its candidates at once (call with all lanes active); 0 for m = 1
+__device__ __forceinline__ uint32_t inv_mod_lanes(uint32_t a, const SmallMod& M, uint32_t lane) {
+  const uint32_t am = M(a);
+  for (uint32_t t0 = 0; t0 < M.m; t0 += 64) {
+    const uint32_t t = t0 + lane;
+    const uint64_t hit = __ballot(t < M.m && M(am * t) == 1u);
+    if (hit) return t0 + (uint32_t)__ffsll((long long)hit) - 1;
+  }
+  return 0;
 }
+
+// 1/d of the power-residue tests' d (setup.cl:22-48), for exact quotients of
+// p - 1 in binary64
+__constant__ double g_prdinv[12] = {1.0 / 3, 1.0 / 4, 1.0 / 5, 1.0 / 7, 1.0 / 8, 1.0 / 9,
+                                    1.0 / 11, 1.0 / 13, 1.0 / 17, 1.0 / 19, 1.0 / 23, 1.0 / 29};
 
 // LDS operations of one wavefront become visible to its other lanes
 __device__ __forceinline__ void wave_sync() {
@@ -343,6 +362,11 @@ struct MontArith {
   // x^e for any 64-bit e (powmod_lg is exact from e = 2 on)
   __device__ V pow_u64(V x, uint64_t e) const { return e <= 1 ? (e ? x : one_) : powmod_lg(x, e, p, q); }
   __device__ void pow_lg2(V x0, V x1, uint64_t e, V* r0, V* r1) const { powmod_lg2(x0, x1, e, p, q, r0, r1); }
+  // d | x, and the quotient (any 64-bit x)
+  __device__ bool divides(uint64_t x, uint32_t d, double, uint64_t* quot) const {
+    *quot = x / d;
+    return *quot * d == x;
+  }
   __device__ uint64_t mont(V x) const { return x; }
   __device__ uint64_t bits(V x) const { return x; }  // a canonical integer key
 };
@@ -366,6 +390,15 @@ struct FpArith {
   __device__ V pow_sm(V x, uint32_t e) const { return fp_powmod_sm(x, e, p, pinv); }
   __device__ V pow_u64(V x, uint64_t e) const { return e <= 1 ? (e ? x : 1.0) : fp_powmod_lg(x, e, p, pinv); }
   __device__ void pow_lg2(V x0, V x1, uint64_t e, V* r0, V* r1) const { fp_powmod_lg2(x0, x1, e, p, pinv, r0, r1); }
+  // d | x, and the quotient, for x < 2^51 (x = p - 1) and dinv = fl(1/d):
+  // rint(x * dinv) is the quotient when d divides x (its relative error is
+  // below 2^-52 and the quotient below 2^51), and fma(-q, d, x), exact, is
+  // zero exactly when it does
+  __device__ bool divides(uint64_t x, uint32_t d, double dinv, uint64_t* quot) const {
+    const double xd = (double)x, qd = rint(xd * dinv);
+    *quot = (uint64_t)qd;
+    return __fma_rn(-qd, (double)d, xd) == 0.0;
+  }
   __device__ uint64_t mont(V x) const { return fp_to_u64(fp_mulmod(x, R, p, pinv)); }
   __device__ uint64_t bits(V x) const { return fp_to_u64(x); }
 };
@@ -385,6 +418,17 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   const uint32_t algo = job->algo;
   const V vone = ar.one(), nmo = ar.neg_one();
   const V mbase = ar.base(base);
+  // x^lane in every lane: an inclusive product scan of (lane ? x : 1), six
+  // products instead of an exponentiation with a per-lane exponent
+  auto lane_pow = [&](V x) {
+    V v = lane ? x : vone;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const V a = shfl_up64(v, d);
+      if ((int)lane >= d) v = ar.mul(v, a);
+    }
+    return v;
+  };
 
   // k for k = lane and k = lane + 64; 'one' pads the products.
   const V mk0 = (int)lane < K ? ar.from_int(job->ak[lane]) : vone;
@@ -429,11 +473,13 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   uint64_t exL = 0;
   if (split) {
     const uint32_t v2 = (uint32_t)__builtin_ctzll(pm);
-    La = (1u << (v2 < 4 ? v2 : 4)) * (pm % 5 == 0 ? 5u : 1u);
-    Lb = (pm % 9 == 0 ? 9u : pm % 3 == 0 ? 3u : 1u) * (pm % 7 == 0 ? 7u : 1u);
-    Lc = (pm % 11 == 0 ? 11u : 1u) * (pm % 13 == 0 ? 13u : 1u);
+    uint64_t qt;
+    La = (1u << (v2 < 4 ? v2 : 4)) * (ar.divides(pm, 5, 1.0 / 5, &qt) ? 5u : 1u);
+    Lb = (ar.divides(pm, 9, 1.0 / 9, &qt) ? 9u : ar.divides(pm, 3, 1.0 / 3, &qt) ? 3u : 1u) *
+         (ar.divides(pm, 7, 1.0 / 7, &qt) ? 7u : 1u);
+    Lc = (ar.divides(pm, 11, 1.0 / 11, &qt) ? 11u : 1u) * (ar.divides(pm, 13, 1.0 / 13, &qt) ? 13u : 1u);
     L = La * Lb * Lc;
-    exL = pm / L;
+    ar.divides(pm, L, 1.0 / (double)L, &exL);
   }
 
   // Lane-parallel exponentiations, one code path (no serialized branches):
@@ -450,18 +496,14 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     ex = p - 2;
     x = total;
   } else if (lane <= 12) {
-    const uint32_t d = g_prd[lane - 1];
-    ex = pm / d;
-    dvalid = (ex * d == pm);
+    dvalid = ar.divides(pm, g_prd[lane - 1], g_prdinv[lane - 1], &ex);
     if (!dvalid) ex = 0;
   } else if (lane == 13) {
     ex = pm >> 1;
-  } else if (split && lane == 14) {
-    ex = pm / La;
-  } else if (split && lane == 15) {
-    ex = pm / Lb;
-  } else if (split && lane == 16 && Lc > 1) {
-    ex = pm / Lc;
+  } else if (split && lane >= 14 && lane <= 16) {
+    // La, Lb, Lc divide p - 1 (one exact quotient for the three lanes)
+    const uint32_t Lz = lane == 14 ? La : lane == 15 ? Lb : Lc;
+    if (Lz > 1 || lane != 16) ar.divides(pm, Lz, 1.0 / (double)Lz, &ex);
   } else if (split && lane == 17) {
     ex = job->nmin;
   }
@@ -594,7 +636,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     // z^i for i <= Lz at zv[vb + i], 64 per pass; the order of z in range
     auto table = [&](V zz, uint32_t Lz, uint32_t vb) -> uint32_t {
       uint32_t sz = 0;
-      V cur = ar.pow_u64(zz, lane);
+      V cur = lane_pow(zz);
       const V z64 = ar.mul(shfl64(cur, 63), zz);
       for (uint32_t j = 0; 64 * j <= Lz; j++) {
         const uint32_t idx = 64 * j + lane;
@@ -608,6 +650,9 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     };
     // (Lc = 1 for most p: then sc = 1 and the third table is not used)
     const uint32_t sa = table(za, La, 0u), sb = table(zb, Lb, 128u), sc = Lc > 1 ? table(zc, Lc, 192u) : 1u;
+    // The CRT needs pairwise coprime orders.  For a prime p they divide 80,
+    // 63 and 143 and are coprime; a composite p may give other orders, which
+    // are then tested by Euclid.
     auto gcd = [](uint32_t g, uint32_t h2) {
       while (h2) {
         const uint32_t t = g % h2;
@@ -617,7 +662,9 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
       return g;
     };
     const uint32_t sab = sa * sb;
-    const uint32_t s = (sa && sb && sc && gcd(sa, sb) == 1 && gcd(sab, sc) == 1) ? sab * sc : 0u;
+    const bool coprime = sa && sb && sc &&
+                         ((80u % sa == 0 && 63u % sb == 0 && 143u % sc == 0) || (gcd(sa, sb) == 1 && gcd(sab, sc) == 1));
+    const uint32_t s = coprime ? sab * sc : 0u;
     wave_sync();
     if (s) {
       for (uint32_t idx = lane; idx < sa; idx += 64) {
@@ -635,7 +682,8 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     }
     wave_sync();
     // sa^-1 mod sb and (sa*sb)^-1 mod sc for the CRT
-    const uint32_t sainv = s ? inv_mod_small(sa, sb) : 0u, sabinv = s ? inv_mod_small(sab, sc) : 0u;
+    const SmallMod Mb(s ? sb : 1u), Mc(s ? sc : 1u);
+    const uint32_t sainv = s ? inv_mod_lanes(sa, Mb, lane) : 0u, sabinv = s ? inv_mod_lanes(sab, Mc, lane) : 0u;
     // index of a key in one table (hash base hb, mask hm, values at vb), or -1
     auto lookup = [&](uint64_t key, uint32_t hb, uint32_t hm, uint32_t vb) -> int {
       for (uint32_t h = (uint32_t)key & hm;; h = (h + 1) & hm) {
@@ -656,10 +704,10 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
         xc = lookup(ar.bits(ycv), 384u, 255u, 192u);
         if (xc < 0) return -1;
       }
-      const uint32_t dx = ((uint32_t)xb + sb - (uint32_t)xa % sb) % sb;
-      const uint32_t xab = (uint32_t)xa + sa * ((dx * sainv) % sb);
-      const uint32_t dy = ((uint32_t)xc + sc - xab % sc) % sc;
-      return (int)(xab + sab * ((dy * sabinv) % sc));
+      const uint32_t dx = Mb((uint32_t)xb + sb - Mb((uint32_t)xa));
+      const uint32_t xab = (uint32_t)xa + sa * Mb(dx * sainv);
+      const uint32_t dy = Mc((uint32_t)xc + sc - Mc(xab));
+      return (int)(xab + sab * Mc(dy * sabinv));
     };
     // Table stride S: s when s is even (it fixes the parity) or no k has a
     // parity class here, else 2s (CRT of the class mod s and the parity);
@@ -695,19 +743,20 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     };
     uint32_t ra[2] = {0, 0}, rb[2] = {0, 0};
     const int na = classes(cls0, ya0, yb0, yc0, ra), nb = classes(cls1, ya1, yb1, yc1, rb);
+    const SmallMod MS(S);
     const uint32_t nmod = job->nmin % S;
     // b^-d for the shifts d < S <= 1441440 < 2^21, in base-64 digits: lane j
     // holds b^-j, b^-(64j), b^-(4096j) and b^-(262144j) (j < 64; the last two
     // levels only when S needs them), so b^-d is one shuffle per digit and a
     // product between (taken with every lane active, outside the divergent
     // stores)
-    const V bl = ar.pow_u64(binv, lane);
+    const V bl = lane_pow(binv);
     const V b64 = ar.mul(shfl64(bl, 63), binv);  // b^-64
-    const V bh = ar.pow_u64(b64, lane);
+    const V bh = lane_pow(b64);
     V b3 = vone, b4 = vone;
     if (S > 4096) {
-      b3 = ar.pow_u64(ar.mul(shfl64(bh, 63), b64), lane);  // (b^-4096)^j
-      if (S > 262144) b4 = ar.pow_u64(ar.mul(shfl64(b3, 63), ar.mul(shfl64(bh, 63), b64)), lane);
+      b3 = lane_pow(ar.mul(shfl64(bh, 63), b64));  // (b^-4096)^j
+      if (S > 262144) b4 = lane_pow(ar.mul(shfl64(b3, 63), ar.mul(shfl64(bh, 63), b64)));
     }
     auto binv_pow = [&](uint32_t d) {
       V r = ar.mul(shfl64(bl, (int)(d & 63u)), shfl64(bh, (int)((d >> 6) & 63u)));
@@ -719,7 +768,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
 #pragma unroll
     for (int a = 0; a < 2; a++) {
       const bool va = na > a, vb = nb > a;
-      const uint32_t da = (ra[a] + S - nmod) % S, db = (rb[a] + S - nmod) % S;
+      const uint32_t da = MS(ra[a] + S - nmod), db = MS(rb[a] + S - nmod);
       const V ta = ar.mul(h0, binv_pow(da)), tb = ar.mul(h1, binv_pow(db));
       const uint64_t m0 = __ballot(va), m1 = __ballot(vb);
       const uint32_t c0 = __popcll(m0), c1 = __popcll(m1);
@@ -741,28 +790,21 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     }
     nk[0] = off;
     // Table length: E* = sqrt(span * targets / (2S)) balances the baby
-    // steps (set, then matched) against the giant-step probes.  The giant
-    // steps are then rounded to F whole rounds of the workgroup (M = F*T,
-    // E = span / (S*M) rounded up), which keeps every probe in the
-    // kernel's pipelined loop; below half a round the balanced shape is
-    // kept.  E is at most the job's Q entries, and a capped table takes as
-    // many giant steps as it needs.
+    // steps (set, then matched) against the giant-step probes.  The BSGS
+    // kernel runs giant steps in blocks of 64 (one per lane), so the giant
+    // steps are rounded to a multiple of 64 (E = span / (S*M) rounded up,
+    // then the M that E needs).  E is at most the job's Q entries, and a
+    // capped table takes as many giant steps as it needs.
     const uint32_t span = job->nmax - job->nmin + 1;
     const double estar = sqrt((double)span * (double)(off + 1) / (2.0 * (double)S));
     const double mstar = (double)span / ((double)S * estar);
-    uint64_t E, M;
-    if (mstar < 0.5 * (double)T) {
-      E = (uint64_t)ceil(estar);
-      M = ((uint64_t)span + (uint64_t)S * E - 1) / ((uint64_t)S * E);
-    } else {
-      const uint64_t F = (uint64_t)rint(mstar / (double)T);
-      E = ((uint64_t)span + (uint64_t)S * F * T - 1) / ((uint64_t)S * F * T);
-      M = F * T;
-    }
-    if (E > job->Q) {
-      E = job->Q;
-      M = ((uint64_t)span + (uint64_t)S * E - 1) / ((uint64_t)S * E);
-    }
+    // ceil(span / d) in binary64, exact for span < 2^53: a quotient just above
+    // an integer n exceeds it by at least 1/d, more than its rounding error
+    auto ceil_div = [&](double d) { return (uint64_t)ceil((double)span / d); };
+    const uint64_t M64 = 64 * (uint64_t)fmax(1.0, rint(mstar / 64.0));
+    uint64_t E = ceil_div((double)S * (double)M64);
+    if (E > job->Q) E = job->Q;
+    const uint64_t M = ceil_div((double)S * (double)E);
     const uint64_t SE = (uint64_t)S * E;
     cS = S;
     cE = (uint32_t)E;

@@ -185,7 +185,9 @@ RS_HD uint64_t rs_as_u64_host(double x) {
 #define RS_AS_U64(x) rs_as_u64_host(x)
 #endif
 
-// a*b - qf*p in (-p, p) (the derivation above)
+// a*b - qf*p in (-p, p) (the derivation above).  The same bounds hold for
+// signed operands |a|, |b| < p (|ab| < p^2 is all they use), so chains of
+// products may stay in (-p, p) and be made canonical once at the end.
 RS_HD double fp_mulmod_signed(double a, double b, double p, double pinv) {
   const double h = RS_FMUL(a, b);
   const double l = RS_FMA(a, b, -h);
@@ -210,31 +212,36 @@ RS_HD uint64_t fp_mulmod_bits(double a, double b, double p, double pinv, double 
 // integer residue of a canonical double
 RS_HD uint64_t fp_to_u64(double c) { return RS_AS_U64(RS_FADD(c, kFpTwo52)) ^ kFpBias; }
 
+// the canonical residue of t in (-p, p)
+RS_HD double fp_canon(double t, double p) { return t < 0.0 ? RS_FADD(t, p) : t; }
+
 // x^exp on plain residues in binary64 with the control flow of powmod_lg
-// (the same masked shift, so the same effective exponent)
+// (the same masked shift, so the same effective exponent); x in (-p, p), the
+// squarings and products stay signed, the result is canonical
 RS_HD double fp_powmod_lg(double x, uint64_t exp, double p, double pinv) {
   uint64_t bit = 0x8000000000000000ull >> ((clz64(exp) + 1) & 63);
   double a = x;
   while (bit) {
-    a = fp_mulmod(a, a, p, pinv);
-    if (exp & bit) a = fp_mulmod(a, x, p, pinv);
+    a = fp_mulmod_signed(a, a, p, pinv);
+    if (exp & bit) a = fp_mulmod_signed(a, x, p, pinv);
     bit >>= 1;
   }
-  return a;
+  return fp_canon(a, p);
 }
 
-// x^exp on plain residues in binary64 with the control flow of powmod_sm
+// x^exp on plain residues in binary64 with the control flow of powmod_sm;
+// canonical result for x in (-p, p)
 RS_HD double fp_powmod_sm(double x, uint32_t exp, double p, double pinv) {
   if (!exp) return 1.0;
-  if (exp == 1) return x;
+  if (exp == 1) return fp_canon(x, p);
   uint32_t bit = 0x80000000u >> ((clz32(exp) + 1) & 31);
   double a = x;
   while (bit) {
-    a = fp_mulmod(a, a, p, pinv);
-    if (exp & bit) a = fp_mulmod(a, x, p, pinv);
+    a = fp_mulmod_signed(a, a, p, pinv);
+    if (exp & bit) a = fp_mulmod_signed(a, x, p, pinv);
     bit >>= 1;
   }
-  return a;
+  return fp_canon(a, p);
 }
 
 // 2^64 mod p for odd p >= 3 (the reference's '(-p) % p').  The quotient of
@@ -249,30 +256,6 @@ RS_HD uint64_t r_mod(uint64_t p) {
   // correction: r = 2^64 - p*qe >= 0 and small multiples of p
   while (r >= p) r -= p;
   return r;
-}
-
-// Jacobi symbol (a/n) for odd n > 0 by the binary algorithm: shifts and
-// subtractions only, no division.  For prime n and a != 0 (mod n) it equals
-// Euler's criterion a^((n-1)/2).
-RS_HD int jacobi(uint64_t a, uint64_t n) {
-  int t = 1;
-  while (a) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    int z = __ffsll((long long)a) - 1;
-#else
-    int z = __builtin_ctzll(a);
-#endif
-    a >>= z;
-    if ((z & 1) && ((n & 7) == 3 || (n & 7) == 5)) t = -t;
-    if (a < n) {
-      uint64_t tmp = a;
-      a = n;
-      n = tmp;
-      if ((a & 3) == 3 && (n & 3) == 3) t = -t;
-    }
-    a -= n;
-  }
-  return n == 1 ? t : 0;
 }
 
 }  // namespace rs

@@ -30,6 +30,41 @@ static void check(uint64_t a, uint64_t b, uint64_t p) {
   }
 }
 
+// Signed operands in (-p, p): the product stays in (-p, p) and is congruent to
+// a*b; the powmods (signed chains, canonical results) equal pow(x, e) mod p.
+static void check_signed(uint64_t a, uint64_t b, uint64_t p, bool na, bool nb) {
+  const double pd = (double)p, pinv = 1.0 / pd;
+  const double sa = na && a ? (double)a - pd : (double)a, sb = nb && b ? (double)b - pd : (double)b;
+  const double t = rs::fp_mulmod_signed(sa, sb, pd, pinv);
+  const uint64_t want = (uint64_t)(((unsigned __int128)a * b) % p);
+  checked++;
+  if (!(t > -pd && t < pd) || rs::fp_to_u64(rs::fp_canon(t, pd)) != want) {
+    if (bad < 10) printf("signed mismatch a=%llu b=%llu p=%llu\n", (unsigned long long)a, (unsigned long long)b,
+                         (unsigned long long)p);
+    bad++;
+  }
+}
+
+static uint64_t powmod128(uint64_t x, uint64_t e, uint64_t p) {
+  unsigned __int128 r = 1, b = x % p;
+  while (e) {
+    if (e & 1) r = r * b % p;
+    b = b * b % p;
+    e >>= 1;
+  }
+  return (uint64_t)r;
+}
+
+static void check_pow(uint64_t x, uint64_t e, uint64_t p) {
+  const double pd = (double)p, pinv = 1.0 / pd;
+  checked++;
+  // exponents >= 2 (an exponent of 1 squares 64 times, the reference's masked shift)
+  if (rs::fp_to_u64(rs::fp_powmod_lg((double)x, e, pd, pinv)) != powmod128(x, e, p)) bad++;
+  checked++;
+  if (e < (1ull << 32) &&
+      rs::fp_to_u64(rs::fp_powmod_sm((double)x, (uint32_t)e, pd, pinv)) != powmod128(x, e, p)) bad++;
+}
+
 int main(int argc, char** argv) {
   const long n = argc > 1 ? atol(argv[1]) : 1000000;
   s_state = argc > 2 ? strtoull(argv[2], 0, 10) : 1;
@@ -52,6 +87,10 @@ int main(int argc, char** argv) {
     check(p - 1, b, p);
     check(0, b, p);
     check(1, b, p);
+    check_signed(a, b, p, i & 1, i & 2);
+    check_signed(p - 1, p - 1, p, true, i & 1);
+    check_signed(1, b, p, true, true);
+    if (i % 16 == 0) check_pow(a, 2 + next_u64() % (p - 2), p);
     // Montgomery residue times plain residue is the Montgomery residue of the product
     const uint64_t q = rs::mont_inv(p), one = rs::r_mod(p);
     const uint64_t am = rs::mmul(a, (uint64_t)(((unsigned __int128)one * one) % p), p, q);  // a*R mod p
