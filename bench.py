@@ -20,9 +20,13 @@ Two paths sieve the same slices of the same terms:
 Their final bitmaps must be identical (checked).  'value' is the tuned path.
 A third leg ('jobs16') sieves sixteen uneven synthetic jobs one at a time
 with the reference split and then all in one batch with the cost model's
-picks, and compares the bitmaps.  With --other-workloads, the small case
-(b=2, K=10, p from 10^9) and the high-p case (b=5, K=100, p from 2^63) are
-timed too, one job each, reference split against the cost model's pick.
+picks, and compares the bitmaps.  A fourth ('high_p') sieves the high-p case
+(b=5, K=100, p from 2^63: integer Montgomery products) with the reference
+split and with the cost model's configuration and split, and reports that
+kernel's multiply rate against the measured v_mad_u64_u32 peak.  Every factor
+drained is checked with the exact 128-bit test.  With --other-workloads, the
+small case (b=2, K=10, p from 10^9) and the high-p case are timed one job
+each, reference split against the cost model's pick.
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -53,6 +57,11 @@ PRESIEVE = 2**20
 # FP64 peak is 78.6 TFLOP/s (256 CUs x 64 lanes x 2 flops x 2.4 GHz).
 FLOPS_PER_MODMUL = 10
 FP64_PEAK = 78.6e12
+# Above 2^51 a product is rs_mont.h's integer Montgomery mmul: 8 v_mad_u64_u32,
+# 4 v_mul_lo_u32 and 2 v_mul_hi_u32 (the gfx950 code of the library), 14
+# 32x32-bit multiply instructions, against the device's measured
+# v_mad_u64_u32 issue rate (rs_peak_mad_u64_u32).
+INT_MULS_PER_MODMUL = 14
 
 
 def make_workload():
@@ -79,7 +88,9 @@ def make_workload():
 
 def cpu_baseline(ks, cs, nmin, nmax, bits, target_seconds=15.0):
     """The CPU oracle (a port of the reference algorithm, oracle/rs_oracle.c)
-    on all host cores, on a bounded sample of the same workload."""
+    threaded over primes on a bounded sample of the same workload.  The
+    threads are the host CPU share of one GPU (16 on the MI355X boxes, whose
+    os.cpu_count() shows the whole host), or all cores of a smaller host."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     nthreads = max(1, min(16, os.cpu_count() or 1))
@@ -99,7 +110,8 @@ def cpu_baseline(ks, cs, nmin, nmax, bits, target_seconds=15.0):
         remaining = target_seconds - spent
         width = int(min(2**28, max(2**18, remaining * rate * 34.7 * 0.8)))
     return {"value": primes / spent, "unit": "primes/s", "cores": nthreads, "kind": "port",
-            "sample": "oracle on p in [2^50+2^38-2^30, +%d): %d primes, %.1f s" % (total_w, primes, spent)}
+            "sample": "oracle on p in [2^50+2^38-2^30, +%d): %d primes, %.1f s; %d threads, the CPU share of one "
+                      "GPU (os.cpu_count() %d)" % (total_w, primes, spent, nthreads, os.cpu_count() or 0)}
 
 
 def copy_calibration(nbytes):
@@ -138,16 +150,45 @@ def roofline(work_mmul, bsgs_s, launches, kernel, path):
     products (the kernel's own count: baby steps, giant-step products,
     giant-step advances) x 10 FP64 flops, per average launch duration (HIP
     events on the library's stream).  The products run on the vector FP64
-    units (this path has no matrix work); MI355X's vector FP64 peak equals its
-    dense FP64 matrix peak, 78.6 TFLOP/s, the compute ('mfma') ceiling."""
+    units (this path has no matrix work; MI355X's vector FP64 peak is 78.6
+    TFLOP/s), so the bound is the VALU."""
     avg = bsgs_s / max(launches, 1)
     per_launch = work_mmul / max(launches, 1)
     achieved = per_launch * FLOPS_PER_MODMUL / max(avg, 1e-12) / 1e12
-    return {"bound": "mfma", "compute": "fp64 VALU", "achieved": achieved, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+    return {"bound": "valu", "compute": "fp64 VALU", "achieved": achieved, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
             "frac": achieved / (FP64_PEAK / 1e12), "traffic": measured_traffic(path, kernel), "kernel": kernel,
             "launches": launches,
             "avg_launch_ms": avg * 1e3, "modmul_per_launch": per_launch,
             "modmul_per_s": per_launch / max(avg, 1e-12)}
+
+
+def roofline_int(work_mmul, bsgs_s, launches, kernel, peak_mad):
+    """The BSGS kernel on the integer Montgomery path (p >= 2^51): its
+    32x32-bit multiply instructions (14 per algorithmic product) per second
+    against the measured v_mad_u64_u32 issue rate of the device."""
+    avg = bsgs_s / max(launches, 1)
+    per_launch = work_mmul / max(launches, 1)
+    achieved = per_launch * INT_MULS_PER_MODMUL / max(avg, 1e-12)
+    return {"bound": "valu", "compute": "int32 multiply (v_mad_u64_u32 / v_mul_lo_u32 / v_mul_hi_u32)",
+            "achieved": achieved / 1e12, "peak": peak_mad / 1e12, "unit": "Tmul/s", "frac": achieved / peak_mad,
+            "traffic": None, "kernel": kernel, "launches": launches, "avg_launch_ms": avg * 1e3,
+            "modmul_per_launch": per_launch, "peak_source": "rs_peak_mad_u64_u32 on this device"}
+
+
+def drain_verified(job, base, cap=1 << 20):
+    """Drain the job's factors and check every one with the exact 128-bit test
+    (rs_verify_factor, outside the timed regions): (factors, exact, composite
+    2-PRPs); a false factor aborts the benchmark."""
+    facs = job.drain(cap)
+    ok = prp = 0
+    for f in facs:
+        k, c = job.sequence(int(f["kidx"]))
+        v = R.verify_factor(int(f["p"]), k, int(f["n"]), c, base)
+        if v == 0:
+            sys.exit("bench.py: %d does not divide %d*%d^%d%+d" % (int(f["p"]), k, base, int(f["n"]), c))
+        ok += v == 1
+        prp += v == -1
+    return len(facs), ok, prp
 
 
 def run_plain(ctx, data, steps, warmup):
@@ -170,40 +211,56 @@ def run_plain(ctx, data, steps, warmup):
     c1 = job.counters()
     bsgs_s, launches, _ = job.kernel_time()
     bm = job.bitmap()
-    job.drain(1 << 20)
+    nf, ok, prp = drain_verified(job, 2)
     job.close()
     M, QQ, MM = R.derive(prm, nmin, nmax)
     return {"primes": c1.primecount - c0.primecount, "seconds": t1 - t0, "bitmap": bm,
             "roof": roofline(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % prm.threads, "plain"),
-            "busy": bsgs_s / (t1 - t0), "claims": c1.claims, "dups": c1.dups,
+            "busy": bsgs_s / (t1 - t0), "claims": c1.claims, "dups": c1.dups, "verified": (nf, ok, prp),
             "cfg": {"algo": "three lists", "Q": prm.Q, "M": M, "QQ": QQ, "MM": MM, "hsize": prm.hsize,
                     "threads": prm.threads, "seg_primes": prm.seg_primes}}
 
 
+def model_choice(ctx, base, kcount, nterms, span, log2p, cands, threads_only=0):
+    """The committed cost model's candidate and batch split through the C ABI
+    (rs_model_choose, rs_model_split: the command line's path), or (None,
+    None) without a model; a model without a split head gives split None."""
+    from rssieve_amd import tuning
+    try:
+        cm = tuning.CModel(ctx)
+    except R.RSError:
+        return None, None
+    try:
+        idx = cm.choose(base, kcount, nterms, span, log2p, cands, threads_only)
+    except R.RSError:
+        cm.close()
+        return None, None
+    try:
+        split = cm.split(base, kcount, nterms, span, log2p)
+    except R.RSError:
+        split = None
+    cm.close()
+    return idx, split
+
+
 def choose_config(ctx, job, data, log):
-    """The tuned configuration: the committed cost model's pick when one is
-    available, else the autotuner's measured best on one sample slice."""
+    """The tuned configuration and batch split: the committed cost model's
+    picks when a model is committed, else the autotuner's measured best on one
+    sample slice and 32 views per round."""
     ks, cs, nmin, nmax, bits, wps = data
     cands = ctx.tune_candidates(nmin, nmax, K)
-    how = "autotune"
-    try:
-        from rssieve_amd import tuning
-        model = tuning.load_model(ctx)
-    except Exception as e:  # no committed model: fall back to measuring
-        model = None
-        log["model_error"] = str(e)[:200]
-    if model is not None and len(cands) == model.n_out:
-        nterms = int(np.bitwise_count(bits).sum())
-        x = tuning.features(2, K, nterms, nmax - nmin + 1, np.log2(float(P0)))[None, :]
-        valid = np.array([[c.threads != 0 for c in cands]])
-        idx = int(model.choose(x, valid)[0])
+    nterms = int(np.bitwise_count(bits).sum())
+    idx, split = model_choice(ctx, 2, K, nterms, nmax - nmin + 1, np.log2(float(P0)), cands)
+    if idx is not None:
         how = "cost model"
         log["model_pick"] = idx
+        log["model_split"] = split
     else:
+        how = "autotune"
         best, secs, primes = job.autotune(P0 + PRANGE - 2**31, 2**26, cands)
         idx = best
         log["autotune_seconds"] = [round(x, 5) for x in secs]
-    return cands[idx], how, idx
+    return cands[idx], how, idx, split
 
 
 def _small_case():
@@ -341,8 +398,15 @@ def run_tuned(ctx, data, steps, warmup, batch):
     prm.factor_cap = 1 << 20
     job = ctx.job(2, ks, cs, nmin, nmax, bits, wps, prm)
     log = {}
-    cfg, how, idx = choose_config(ctx, job, data, log)
+    cfg, how, idx, split = choose_config(ctx, job, data, log)
     cfg.factor_cap = 1 << 20
+    split_by = "option"
+    if not batch:
+        if split is not None:
+            batch, cfg.seg_primes = split
+            split_by = "cost model"
+        else:
+            batch = 32
     job.reconfigure(cfg)
     views = [job.view() for _ in range(batch - 1)]
     handles = [job] + views
@@ -367,17 +431,72 @@ def run_tuned(ctx, data, steps, warmup, batch):
     c1 = job.counters()
     bsgs_s, launches, _ = job.kernel_time()
     bm = job.bitmap()
-    job.drain(1 << 20)
+    nf, ok, prp = drain_verified(job, 2)
     for v in views:
         v.close()
     job.close()
     return {"primes": c1.primecount - c0.primecount, "seconds": t1 - t0, "bitmap": bm,
             "roof": roofline(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % cfg.threads, "tuned"),
-            "busy": bsgs_s / (t1 - t0), "claims": c1.claims, "dups": c1.dups,
+            "busy": bsgs_s / (t1 - t0), "claims": c1.claims, "dups": c1.dups, "verified": (nf, ok, prp),
             "cfg": {"algo": ALGO_NAMES[cfg.algo], "Q": cfg.Q,
                     "hsize": cfg.hsize, "threads": cfg.threads, "seg_primes": cfg.seg_primes, "batch": batch,
-                    "chosen_by": how, "candidate": idx},
+                    "chosen_by": how, "split_by": split_by, "candidate": idx},
             "log": log}
+
+
+def run_high_p(ctx, peak_mad, width=2**28):
+    """The high-p case on the integer Montgomery path: b=5, K=100, n in
+    [10^6, 2^24) (1/32 of the terms), p in [2^63, 2^63 + width): the reference
+    split (one job, one view) and the cost model's configuration and split
+    (batched views), same final bitmap; the tuned kernel's roofline is its
+    32x32-bit multiply rate against the measured v_mad_u64_u32 peak."""
+    ks, cs, nmin, nmax, bm, wps = _high_p_case()
+    p0 = 2**63
+    base = 5
+    nterms = int(np.bitwise_count(bm).sum())
+    cands = ctx.tune_candidates(nmin, nmax, len(ks))
+    idx, split = model_choice(ctx, base, len(ks), nterms, nmax - nmin + 1, 63.0, cands)
+    ref = R.reference_params(nmin, nmax, len(ks), 65536, ctx.compute_units)
+    out = {}
+    for label, prm, nv in (("plain", ref, 1), ("tuned", cands[idx if idx is not None else 0], 32)):
+        prm.factor_cap = 1 << 16
+        if label == "tuned" and split is not None:
+            nv, prm.seg_primes = split
+        job = ctx.job(base, ks, cs, nmin, nmax, bm.copy(), wps, prm)
+        views = [job.view() for _ in range(nv - 1)]
+        hs = [job] + views
+        part = width // nv
+        lo = [p0 + i * part for i in range(nv)]
+        hi = [p0 + (i + 1) * part if i < nv - 1 else p0 + width for i in range(nv)]
+        warm = [x + 1 for x in lo]  # the batch shape's graphs, outside the timed region
+        R.sieve_batch(hs, lo, warm)
+        ctx.sync()
+        c0 = job.counters()
+        job.timer_reset()
+        t0 = time.perf_counter()
+        R.sieve_batch(hs, [x + 1 for x in lo], hi)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        c1 = job.counters()
+        bsgs_s, launches, _ = job.kernel_time()
+        nf, ok, prp = drain_verified(job, base, 1 << 16)
+        out[label] = {"primes": c1.primecount - c0.primecount, "seconds": dt, "bitmap": job.bitmap(),
+                      "roof": roofline_int(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % prm.threads,
+                                           peak_mad),
+                      "busy": bsgs_s / dt, "verified": ok, "algo": ALGO_NAMES[prm.algo], "Q": prm.Q,
+                      "threads": prm.threads, "views": nv, "seg_primes": prm.seg_primes}
+        for v in views:
+            v.close()
+        job.close()
+    same = bool(np.array_equal(out["plain"]["bitmap"], out["tuned"]["bitmap"]))
+    pl, tu = out["plain"], out["tuned"]
+    return {"workload": "b=5, K=100, n in [1e6,2^24) (1/32 of the terms), p in [2^63, 2^63+2^28)",
+            "plain_primes_per_s": pl["primes"] / pl["seconds"], "tuned_primes_per_s": tu["primes"] / tu["seconds"],
+            "tuned_over_plain": pl["seconds"] / tu["seconds"] * tu["primes"] / max(pl["primes"], 1),
+            "same_terms": same, "tuned_config": {k: tu[k] for k in ("algo", "Q", "threads", "views", "seg_primes")},
+            "candidate": idx, "factors_verified": pl["verified"],
+            "roofline": dict(tu["roof"], bsgs_busy_frac=tu["busy"]),
+            "plain_roofline": dict(pl["roof"], bsgs_busy_frac=pl["busy"])}
 
 
 def main():
@@ -385,7 +504,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=32, help="views of the job sieved per batched round")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="views of the job per batched round (default: the cost model's split, else 32)")
+    ap.add_argument("--no-high-p", action="store_true", help="skip the high-p (integer Montgomery) leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--tuned-only", action="store_true", help="skip the plain leg (for profiling the tuned kernel)")
@@ -425,6 +546,12 @@ def main():
     if not args.no_jobs16 and not args.plain_only and not args.tuned_only:
         jobs16 = run_jobs16(ctx)
     other = run_other_workloads(ctx) if args.other_workloads else None
+    high_p = None
+    if not args.no_high_p and not args.plain_only and not args.tuned_only:
+        peak_mad = ctx.peak_mad_u64_u32()
+        high_p = run_high_p(ctx, peak_mad)
+        if not high_p["same_terms"]:
+            sys.exit("bench.py: high-p tuned and plain paths eliminated different terms")
     ctx.close()
     value = res["primes"] / res["seconds"]
     line = {
@@ -451,6 +578,7 @@ def main():
         "same_terms": tuned is not None and not args.tuned_only,
         "primes_per_step": res["primes"] / args.steps,
         "factors": int(res["claims"]),
+        "factors_verified": int(res["verified"][1]),
         "setup_s": setup_s,
     }
     if tuned:
@@ -459,6 +587,8 @@ def main():
         line["jobs16"] = jobs16
     if other is not None:
         line["other_workloads"] = other
+    if high_p is not None:
+        line["high_p"] = high_p
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(*data[:5])
     print(json.dumps(line))

@@ -156,7 +156,13 @@ int rs_sieve(rs_job *job, uint64_t pmin, uint64_t pmax);
 #define RS_NOALLOC_SCRATCH_BYTES (48u << 20)
 int rs_sieve_noalloc(rs_job *job, uint64_t pmin, uint64_t pmax);
 /* Enqueue several jobs into one persistent BSGS launch per segment round
- * (no reference counterpart: the reference runs one job per process). */
+ * (no reference counterpart: the reference runs one job per process).  The
+ * launch has one shape: every job needs the same BSGS workgroup size
+ * (rs_params.threads, else RS_E_ARG); the LDS holds the largest filter of the
+ * batch, and the workgroups per CU are the occupancy maximum, capped by the
+ * smallest nonzero blocks_per_cu of the jobs.  A job handle may appear once
+ * (RS_E_ARG otherwise): several p-ranges of one job go in as views
+ * (rs_job_view). */
 int rs_sieve_batch(rs_job **jobs, int njobs, const uint64_t *pmin, const uint64_t *pmax);
 /* Wait for everything enqueued on the context. */
 int rs_sync(rs_ctx *ctx);
@@ -181,6 +187,15 @@ int rs_job_timer_reset(rs_job *job);
 /* Every prime the generator emits for [pmin, pmax) (getsegprimes.cl +
  * addsmallprimes.cl), unordered.  *count receives the total (<= cap copied). */
 int rs_primegen(rs_ctx *ctx, uint64_t pmin, uint64_t pmax, uint64_t *out, uint64_t cap, uint64_t *count);
+
+/* --- measurement --------------------------------------------------------- */
+
+/* Issue rate of v_mad_u64_u32 on the context's device, in lane operations per
+ * second (independent multiply-add chains on every lane of 16 workgroups per
+ * compute unit, best of three timed launches): the ceiling of the integer
+ * Montgomery path's roofline (a product issues 8 of them, 4 v_mul_lo_u32 and
+ * 2 v_mul_hi_u32).  No counterpart in the reference. */
+int rs_peak_mad_u64_u32(rs_ctx *ctx, double *lane_ops_per_s);
 
 /* --- host-side exact arithmetic (verify_factor.c) ---------------------- */
 

@@ -38,6 +38,17 @@ int rs_autotune(rs_job *job, uint64_t pmin, uint64_t width, const rs_params *can
  * workgroup size, segment target); the bitmap and counters are kept. */
 int rs_job_reconfigure(rs_job *job, const rs_params *prm);
 
+/* The batch split of a job (no reference counterpart; the reference sizes one
+ * segment per round, primes_per_bsgs, cl_sieve.cpp:693-735): option i sieves
+ * [pmin, pmin + width) as views[i] equal parts in one batch (rs_job_view,
+ * rs_sieve_batch) with seg_primes[i] target primes per segment, so one
+ * persistent BSGS launch serves views[i] * seg_primes[i] primes.
+ * seconds[i] = stream time, primes[i] = primes of the sample; *best = the
+ * option with the fewest seconds per prime.  Bitmap and counters are
+ * restored and the job keeps its configuration. */
+int rs_autotune_split(rs_job *job, uint64_t pmin, uint64_t width, const uint32_t *views, const uint32_t *seg_primes,
+                      int nopt, double *seconds, uint64_t *primes, int *best);
+
 /* --- learned cost model ----------------------------------------------------- */
 
 typedef struct rs_mlp rs_mlp;
@@ -63,6 +74,31 @@ int rs_mlp_adam_step(rs_mlp *m, float lr);
  * over the whole set afterwards. */
 int rs_mlp_train(rs_mlp *m, const float *X, const float *T, const float *M, int n, int epochs, int batch, float lr,
                  uint64_t seed, float *final_loss);
+
+/* --- the trained model from C ----------------------------------------------- */
+
+/* A cost model file written by the training script (rssieve_amd/tuning.py;
+ * the committed one is rssieve_amd/model/cost_model.bin): a head over the
+ * BSGS candidates of rs_tune_candidates and, optionally, a head over batch
+ * splits, each an MLP whose parameters are loaded into rs_mlp objects on ctx
+ * and evaluated by the library's kernels.  RS_E_IO on a missing or malformed
+ * file. */
+typedef struct rs_model rs_model;
+int rs_model_load(rs_ctx *ctx, const char *path, rs_model **out);
+void rs_model_free(rs_model *m);
+/* The candidate (index into cand, the list rs_tune_candidates gives for the
+ * job's shape) with the lowest predicted seconds per prime for a job of
+ * 'kcount' sequences of base 'base', 'nterms' candidate terms over an n-span
+ * of 'span', sieved at p ~ 2^log2p; only candidates with threads != 0 (and
+ * threads == threads_only, if nonzero) are eligible; 0, the reference split,
+ * if none is.  RS_E_ARG if ncand differs from the model's candidate count. */
+int rs_model_choose(rs_model *m, uint32_t base, int32_t kcount, uint64_t nterms, uint32_t span, double log2p,
+                    const rs_params *cand, int ncand, uint32_t threads_only, int *idx);
+/* The batch split with the lowest predicted seconds per prime for the same
+ * job: views per batched round and target primes per segment (the options
+ * rs_autotune_split times).  RS_E_ARG if the model has no split head. */
+int rs_model_split(rs_model *m, uint32_t base, int32_t kcount, uint64_t nterms, uint32_t span, double log2p,
+                   uint32_t *views, uint32_t *seg_primes);
 
 #ifdef __cplusplus
 }

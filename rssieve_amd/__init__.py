@@ -113,6 +113,7 @@ _SIGS = {
     "rs_job_timer_reset": (C.c_int, [C.c_void_p]),
     "rs_primegen": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64), C.c_uint64,
                               C.POINTER(C.c_uint64)]),
+    "rs_peak_mad_u64_u32": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "rs_verify_factor": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int32, C.c_uint32]),
     "rs_is_prime": (C.c_int, [C.c_uint64]),
     "rs_wheel_offset": (None, [C.POINTER(C.c_uint64), C.POINTER(C.c_int32)]),
@@ -137,6 +138,15 @@ _SIGS = {
     # rssieve_mlp.h
     "rs_tune_candidates": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(Params), C.c_int,
                                      C.POINTER(C.c_int)]),
+    "rs_model_load": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    "rs_model_free": (None, [C.c_void_p]),
+    "rs_model_choose": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, C.c_uint64, C.c_uint32, C.c_double,
+                                  C.POINTER(Params), C.c_int, C.c_uint32, C.POINTER(C.c_int)]),
+    "rs_model_split": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, C.c_uint64, C.c_uint32, C.c_double,
+                                 C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "rs_autotune_split": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_int)]),
     "rs_autotune": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(Params), C.c_int,
                               C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.POINTER(C.c_int)]),
     "rs_job_reconfigure": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
@@ -318,6 +328,12 @@ class Context:
         _check(lib().rs_primegen(self._h, pmin, pmax, _u64p(out), cap, C.byref(cnt)))
         return out[:min(cnt.value, cap)]
 
+    def peak_mad_u64_u32(self):
+        """Measured v_mad_u64_u32 lane operations per second (rs_peak_mad_u64_u32)."""
+        v = C.c_double()
+        _check(lib().rs_peak_mad_u64_u32(self._h, C.byref(v)))
+        return v.value
+
     def tune_candidates(self, nmin, nmax, kcount):
         cap = 128
         arr = (Params * cap)()
@@ -426,6 +442,19 @@ class Job:
         pr = (C.c_uint64 * n)()
         best = C.c_int()
         _check(lib().rs_autotune(self._h, pmin, width, arr, n, secs, pr, C.byref(best)))
+        return best.value, list(secs), list(pr)
+
+    def autotune_split(self, pmin, width, views, seg_primes):
+        """rs_autotune_split: (best option, seconds, primes) of each (views,
+        seg_primes) option over the same sample."""
+        n = len(views)
+        assert len(seg_primes) == n
+        v = (C.c_uint32 * n)(*views)
+        sp = (C.c_uint32 * n)(*seg_primes)
+        secs = (C.c_double * n)()
+        pr = (C.c_uint64 * n)()
+        best = C.c_int()
+        _check(lib().rs_autotune_split(self._h, pmin, width, v, sp, n, secs, pr, C.byref(best)))
         return best.value, list(secs), list(pr)
 
     def timer_reset(self):

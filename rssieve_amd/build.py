@@ -22,7 +22,7 @@ ARCH = "gfx950"
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include")]
 
-LIB_SOURCES = ["rs_kernels.hip", "rs_mlp.hip", "rs_capi.cpp", "rs_host.cpp", "rs_io.cpp"]
+LIB_SOURCES = ["rs_kernels.hip", "rs_mlp.hip", "rs_capi.cpp", "rs_host.cpp", "rs_io.cpp", "rs_model.cpp", "rs_peak.hip"]
 CLI_SOURCES = ["rs_cli.cpp"]
 
 

@@ -565,10 +565,19 @@ extern "C" int rs_sieve_batch(rs_job** jobs, int njobs, const uint64_t* pmin, co
   uint64_t R = 0;
   std::vector<uint64_t> ranges(njobs);
   std::vector<uint32_t> gbs(njobs);
+  uint32_t bpc_cap = 0;  // the smallest blocks_per_cu a job asks for (0: none)
   for (int i = 0; i < njobs; i++) {
     rs_job* j = jobs[i];
     if (!j || j->ctx != ctx) return fail(RS_E_ARG, "rs_sieve_batch: jobs must share one context");
     if (pmin[i] < 3 || pmax[i] < pmin[i]) return fail(RS_E_ARG, "rs_sieve_batch: bad range of job %d", i);
+    if (j->h.threads != T)
+      return fail(RS_E_ARG, "rs_sieve_batch: job %d has %u-thread BSGS workgroups, job 0 has %u (one launch shape "
+                  "per batch)", i, j->h.threads, T);
+    for (int k = 0; k < i; k++)
+      if (jobs[k] == j)
+        return fail(RS_E_ARG, "rs_sieve_batch: job %d appears twice (a job has one segment buffer; sieve more "
+                    "ranges of it through rs_job_view)", i);
+    if (j->prm.blocks_per_cu && (!bpc_cap || j->prm.blocks_per_cu < bpc_cap)) bpc_cap = j->prm.blocks_per_cu;
     uint64_t range;
     uint32_t gb;
     segment_shape(j, pmin[i], &range, &gb);
@@ -582,6 +591,7 @@ extern "C" int rs_sieve_batch(rs_job** jobs, int njobs, const uint64_t* pmin, co
   if (bsgs_block_lds(hsize, T) > ctx->lds_per_block) return fail(RS_E_ARG, "rs_sieve_batch: table too large");
   int bpc = 0;
   if (bsgs_blocks_per_cu(T, hsize, &bpc) != hipSuccess || bpc < 1) bpc = 1;
+  if (bpc_cap && (int)bpc_cap < bpc) bpc = (int)bpc_cap;
   const uint32_t bsgs_blocks = (uint32_t)bpc * ctx->cus;
   const uint32_t setup_blocks = (cap_max + 3) / 4;
   HIPCHK(hipSetDevice(ctx->device));
@@ -1009,4 +1019,99 @@ extern "C" int rs_autotune(rs_job* j, uint64_t pmin, uint64_t width, const rs_pa
   *best = ib;
   if (ib >= 0) return rs_job_reconfigure(j, &cand[ib]);
   return rs_job_reconfigure(j, &orig);
+}
+
+// The batch split of one job: for each option i the job is reconfigured to
+// seg_primes[i] target primes per segment and [pmin, pmin + width) is cut
+// into views[i] equal parts sieved as one batch (the job and views[i] - 1
+// views, one persistent BSGS launch per round), timed with stream events;
+// the bitmap and counters are restored after every option, the views
+// destroyed, and the job left on its original configuration.
+extern "C" int rs_autotune_split(rs_job* j, uint64_t pmin, uint64_t width, const uint32_t* views,
+                                 const uint32_t* seg_primes, int nopt, double* seconds, uint64_t* primes, int* best) {
+  if (!j || !views || !seg_primes || nopt < 1 || !seconds || !primes || !best || width == 0 || pmin < 3)
+    return fail(RS_E_ARG, "rs_autotune_split: bad args");
+  if (j->parent) return fail(RS_E_ARG, "rs_autotune_split: tune the parent job, not a view");
+  for (int i = 0; i < nopt; i++)
+    if (views[i] < 1 || views[i] > RS_MAX_BATCH || seg_primes[i] < 1024)
+      return fail(RS_E_ARG, "rs_autotune_split: option %d: views %u (1..%d), seg_primes %u (>= 1024)", i, views[i],
+                  RS_MAX_BATCH, seg_primes[i]);
+  HIPCHK(hipSetDevice(j->ctx->device));
+  hipStream_t s = j->ctx->stream;
+  HIPCHK(hipStreamSynchronize(s));
+  uint32_t* saved_bm = nullptr;
+  HIPCHK(hipMalloc((void**)&saved_bm, j->nwords * 4));
+  Counters saved_cnt;
+  int rc = RS_OK;
+  if (hipMemcpy(saved_bm, j->dbitmap, j->nwords * 4, hipMemcpyDeviceToDevice) != hipSuccess ||
+      hipMemcpy(&saved_cnt, j->dcnt, sizeof(Counters), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(RS_E_HIP, "rs_autotune_split: snapshot");
+  const rs_params orig = j->prm;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(RS_E_HIP, "rs_autotune_split: events");
+  const uint64_t pmax = (pmin + width < pmin) ? ~0ull : pmin + width;
+  int ib = -1;
+  double tb = 0;
+  std::vector<rs_job*> handles;
+  std::vector<uint64_t> lo, hi;
+  for (int i = -1; i < nopt && !rc; i++) {
+    // i == -1: one untimed pass of the first option (code-object load, caches)
+    const int o = i < 0 ? 0 : i;
+    rs_params prm = orig;
+    prm.seg_primes = seg_primes[o];
+    rc = rs_job_reconfigure(j, &prm);
+    if (rc) break;
+    const uint32_t V = views[o];
+    handles.assign(1, j);
+    for (uint32_t v = 1; v < V && !rc; v++) {
+      rs_job* h = nullptr;
+      rc = rs_job_view(j, &h);
+      if (!rc) handles.push_back(h);
+    }
+    if (!rc) {
+      lo.resize(V);
+      hi.resize(V);
+      const uint64_t part = (pmax - pmin) / V;
+      for (uint32_t v = 0; v < V; v++) {
+        lo[v] = pmin + v * part;
+        hi[v] = v + 1 < V ? pmin + (v + 1) * part : pmax;
+      }
+      // capture this batch shape's graphs outside the timed region: one round
+      // over a one-integer range per view has the same launch shape
+      std::vector<uint64_t> lo1(lo);
+      for (uint32_t v = 0; v < V; v++) lo1[v] = lo[v] + 1;
+      rc = rs_sieve_batch(handles.data(), (int)V, lo.data(), lo1.data());
+      if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = fail(RS_E_HIP, "rs_autotune_split: event");
+      if (!rc) rc = rs_sieve_batch(handles.data(), (int)V, lo.data(), hi.data());
+      if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+        rc = fail(RS_E_HIP, "rs_autotune_split: sync");
+    }
+    Counters c;
+    if (!rc && hipMemcpy(&c, j->dcnt, sizeof(Counters), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(RS_E_HIP, "rs_autotune_split: counters");
+    float ms = 0;
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(RS_E_HIP, "rs_autotune_split: timing");
+    for (size_t v = 1; v < handles.size(); v++) rs_job_destroy(handles[v]);
+    handles.clear();
+    if (!rc && (hipMemcpy(j->dbitmap, saved_bm, j->nwords * 4, hipMemcpyDeviceToDevice) != hipSuccess ||
+                hipMemcpy(j->dcnt, &saved_cnt, sizeof(Counters), hipMemcpyHostToDevice) != hipSuccess))
+      rc = fail(RS_E_HIP, "rs_autotune_split: restore");
+    if (rc || i < 0) continue;
+    seconds[i] = ms * 1e-3;
+    primes[i] = c.primecount - saved_cnt.primecount;
+    if (ib < 0 || seconds[i] / std::max<double>(1.0, (double)primes[i]) < tb) {
+      ib = i;
+      tb = seconds[i] / std::max<double>(1.0, (double)primes[i]);
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(saved_bm);
+  j->ev_used = 0;
+  j->timer_started = false;
+  const int rc2 = rs_job_reconfigure(j, &orig);
+  if (rc) return rc;
+  *best = ib;
+  return rc2;
 }

@@ -14,6 +14,8 @@
 #include <string>
 #include <vector>
 
+#include <unistd.h>
+
 #include "rssieve.h"
 #include "rssieve_io.h"
 #include "rssieve_mlp.h"
@@ -25,8 +27,11 @@ struct Options {
   std::string input, factor_file;
   bool selftest = false;
   int device = 0;
-  bool merged = false;  // -t: the merged-table BSGS
-  bool compat = false;  // -c: reference signed-int k encoding
+  bool autotune = false;  // -t: time every candidate configuration on a sample, sieve with the fastest
+  bool learned = false;   // -l / -m: the cost model picks the configuration and the batch split
+  std::string model_path; // -m: model file (default: model/cost_model.bin beside the executable)
+  uint32_t views = 0;     // -b: views per batched round (0: 1, or the model's split with -l)
+  bool compat = false;    // -c: reference signed-int k encoding
 };
 
 void help() {
@@ -38,7 +43,10 @@ void help() {
   printf("-i inputfile\t\tUse specified sr2sieve ABCD input file with a maximum of 100 sequences\n");
   printf("-f factorfile\t\tOverride default factor file name (psp_sr2sieve.out) with specified file name.\n");
   printf("-d #\t\t\tHIP device index (default 0)\n");
-  printf("-t\t\t\tUse the merged-table BSGS with tuned parameters\n");
+  printf("-t\t\t\tAutotune: time every candidate BSGS configuration on a sample and use the fastest\n");
+  printf("-l\t\t\tLearned: the committed cost model picks the BSGS configuration and the batch split\n");
+  printf("-m modelfile\t\tLike -l with the given cost model file\n");
+  printf("-b #\t\t\tSieve as # views of the range per batched round (one persistent launch for all)\n");
   printf("-c\t\t\tEncode k as the reference's signed int (k >= 2^31 wraps)\n");
   printf("-s\t\t\tRun the built-in self test\n");
   printf("-h\t\t\tPrint this help\n");
@@ -106,7 +114,23 @@ void parse_cmdline(int argc, char** argv, Options* o) {
     } else if (!strcmp(tok, "-s")) {
       o->selftest = true;
     } else if (!strcmp(tok, "-t")) {
-      o->merged = true;
+      o->autotune = true;
+    } else if (!strcmp(tok, "-l")) {
+      o->learned = true;
+    } else if (!strcmp(tok, "-m")) {
+      tok = strtok_r(nullptr, " \t", &save);
+      if (tok) {
+        o->model_path = tok;
+        o->learned = true;
+      }
+    } else if (!strcmp(tok, "-b")) {
+      tok = strtok_r(nullptr, " \t", &save);
+      uint64_t v = 0;
+      if (tok && parse_uint64(&v, tok, 1, 256) == 0) o->views = (uint32_t)v;
+      else if (tok) {
+        fprintf(stderr, "Invalid value for -b: %s\n", tok);
+        printf("Invalid value for -b: %s\n", tok);
+      }
     } else if (!strcmp(tok, "-c")) {
       o->compat = true;
     } else if (!strcmp(tok, "-h")) {
@@ -180,11 +204,27 @@ void print_progress(const rs_state& st, double* smooth, time_t start_time, uint6
   fflush(stdout);
 }
 
+// model/cost_model.bin in the directory of this executable
+std::string default_model_path() {
+  char buf[4096];
+  const ssize_t n = readlink("/proc/self/exe", buf, sizeof(buf) - 1);
+  if (n <= 0) return "model/cost_model.bin";
+  buf[n] = '\0';
+  std::string d(buf);
+  const size_t slash = d.rfind('/');
+  return (slash == std::string::npos ? std::string(".") : d.substr(0, slash)) + "/model/cost_model.bin";
+}
+
+const char* algo_name(int32_t a) {
+  return a == RS_ALGO_CLASSES ? "residue-class tables" : a == RS_ALGO_MERGED ? "merged table" : "three lists";
+}
+
 struct Run {
   Options o;
   rs_abcd* abcd = nullptr;
   rs_ctx* ctx = nullptr;
   rs_job* job = nullptr;
+  std::vector<rs_job*> views;  // the job's extra views of a batched run
   rs_state st{};
   int write_a_next = 1;
   uint64_t factor_cap = 0;
@@ -413,7 +453,45 @@ int sieve(Options o) {
   if (rs_job_create(r.ctx, base, kcount, ks.data(), cs.data(), nmin, nmax, rs_abcd_bitmap(r.abcd), wps, &prm,
                     &r.job))
     die_lib("rs_job_create");
-  if (o.merged && st.pmax - st.p > 4096) {
+  uint32_t nviews = o.views;
+  if (o.learned) {
+    // the cost model's configuration and batch split (cl_sieve.cpp:885-973's
+    // fixed rule stays the fallback)
+    const std::string path = o.model_path.empty() ? default_model_path() : o.model_path;
+    rs_model* model = nullptr;
+    if (rs_model_load(r.ctx, path.c_str(), &model)) {
+      fprintf(stderr, "warning: %s; using the reference configuration\n", rs_last_error());
+      printf("warning: %s; using the reference configuration\n", rs_last_error());
+    } else {
+      std::vector<rs_params> cand(128);
+      int nc = 0;
+      if (rs_tune_candidates(r.ctx, nmin, nmax, kcount, cand.data(), (int)cand.size(), &nc)) die_lib("rs_tune_candidates");
+      nc = std::min<int>(nc, (int)cand.size());
+      const uint32_t span = nmax - nmin + 1;
+      const double log2p = std::log2((double)st.p);
+      int idx = 0;
+      if (rs_model_choose(model, base, kcount, nterms, span, log2p, cand.data(), nc, 0, &idx)) {
+        fprintf(stderr, "warning: %s; using the reference configuration\n", rs_last_error());
+        idx = 0;
+      }
+      rs_params pick = cand[idx];
+      pick.k31_compat = prm.k31_compat;
+      pick.factor_cap = prm.factor_cap;
+      uint32_t mv = 0, msp = 0;
+      if (rs_model_split(model, base, kcount, nterms, span, log2p, &mv, &msp) == RS_OK) {
+        pick.seg_primes = msp;
+        if (!nviews) nviews = mv;
+      } else if (!nviews) {
+        nviews = 32;  // no split head: the benchmark's batch
+      }
+      if (rs_job_reconfigure(r.job, &pick)) die_lib("rs_job_reconfigure");
+      prm = pick;
+      printf("Cost model: candidate %d of %d (%s), %u target primes per segment\n", idx, nc, algo_name(prm.algo),
+             prm.seg_primes);
+      rs_model_free(model);
+    }
+  }
+  if (o.autotune && st.pmax - st.p > 4096) {
     // -t: time every candidate configuration on a short sample at the start of
     // the range (the job's bitmap and counters are restored afterwards)
     std::vector<rs_params> cand(128);
@@ -433,8 +511,20 @@ int sieve(Options o) {
     int best = 0;
     if (rs_autotune(r.job, st.p, width, cand.data(), nc, secs.data(), pr.data(), &best)) die_lib("rs_autotune");
     prm = cand[best];
-    printf("Autotuned: candidate %d of %d (%s)\n", best, nc, prm.algo == RS_ALGO_CLASSES ? "residue-class tables" : prm.algo == RS_ALGO_MERGED ? "merged table" : "three lists");
+    printf("Autotuned: candidate %d of %d (%s)\n", best, nc, algo_name(prm.algo));
   }
+  if (!nviews) nviews = 1;
+  // a batched run: the range of every round cut into nviews parts, one per
+  // view of the job (they share its bitmap, counters and factor list)
+  for (uint32_t v = 1; v < nviews; v++) {
+    rs_job* h = nullptr;
+    if (rs_job_view(r.job, &h)) die_lib("rs_job_view");
+    r.views.push_back(h);
+  }
+  std::vector<rs_job*> handles(1, r.job);
+  handles.insert(handles.end(), r.views.begin(), r.views.end());
+  std::vector<uint64_t> vlo(nviews), vhi(nviews);
+  if (nviews > 1) printf("Batched: %u views per round\n", nviews);
   uint32_t M, QQ, MM;
   rs_params_derive(&prm, nmin, nmax, &M, &QQ, &MM);
   printf("Q:%u M:%u QQ:%u MM:%u HT:%u LS:%u\n", prm.Q, M, QQ, MM, prm.hsize, prm.threads);
@@ -455,7 +545,16 @@ int sieve(Options o) {
     uint64_t stop = st.p + chunk;
     if (stop > st.pmax || stop < st.p) stop = st.pmax;
     const auto c0 = std::chrono::steady_clock::now();
-    if (rs_sieve(r.job, st.p, stop)) die_lib("rs_sieve");
+    if (nviews == 1) {
+      if (rs_sieve(r.job, st.p, stop)) die_lib("rs_sieve");
+    } else {
+      const uint64_t part = (stop - st.p) / nviews;
+      for (uint32_t v = 0; v < nviews; v++) {
+        vlo[v] = st.p + v * part;
+        vhi[v] = v + 1 < nviews ? st.p + (v + 1) * part : stop;
+      }
+      if (rs_sieve_batch(handles.data(), (int)nviews, vlo.data(), vhi.data())) die_lib("rs_sieve_batch");
+    }
     rs_counters c;
     if (rs_counters_read(r.job, &c)) die_lib("rs_counters_read");
     const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
@@ -479,8 +578,10 @@ int sieve(Options o) {
   fprintf(stderr, "Sieve complete.\nfactors %" PRIu64 ", prime count %" PRIu64 ", duplicate factors %" PRIu64 "\n",
           st.factorcount, st.primecount, st.dupcount);
   printf("Sieve finished in %d sec.\n", (int)secs);
+  printf("Primes tested per second: %.0f\n", secs > 0 ? (double)(st.primecount - r.base_primecount) / secs : 0.0);
   printf("factors %" PRIu64 ", prime count %" PRIu64 ", checksum %016" PRIX64 ", duplicate factors %" PRIu64 "\n",
          st.factorcount, st.primecount, st.checksum, st.dupcount);
+  for (rs_job* v : r.views) rs_job_destroy(v);
   rs_job_destroy(r.job);
   rs_close(r.ctx);
   rs_abcd_free(r.abcd);

@@ -43,7 +43,8 @@ int mfail(int code, const char* fmt, ...) {
   } while (0)
 
 // out[i][j] = act(sum_k in[i][k] W[k][j] + b[j]); one thread per (i, j),
-// the weight column staged through LDS in tiles of 64 rows.
+// reading its weight column straight from global memory (the layers are at
+// most 64 wide, so the column stays in cache).
 __global__ void k_dense_fwd(const float* __restrict__ in, const float* __restrict__ W, const float* __restrict__ b,
                             float* __restrict__ z, float* __restrict__ out, int n, int a, int bw, int relu) {
   const int j = blockIdx.y * blockDim.x + threadIdx.x;  // output unit

@@ -9,10 +9,6 @@ for p in (ROOT, os.path.join(ROOT, "oracle")):
         sys.path.insert(0, p)
 
 
-def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built librssieve.so")
-
-
 @pytest.fixture(scope="session")
 def ctx():
     import rssieve_amd as R

@@ -106,3 +106,101 @@ def test_cli_tuned_same_terms(tmp_path):
             terms.add((k, s, n))
         outs.append(terms)
     assert outs[0] == outs[1]
+
+
+def _terms(path, base):
+    out = []
+    for l in _factor_lines(path):
+        if l == "no factors":
+            continue
+        p, k, s, n = R.parse_factor(l, base)
+        out.append((p, k, 1 if s == "+" else -1, n))
+    return out
+
+
+def test_cli_learned_batched_same_terms_as_plain_and_oracle(tmp_path):
+    """-l (the committed cost model picks the configuration and the split,
+    through the C ABI) with -b 4 (four views per batched round): the same
+    eliminated terms as the plain run (reference split, one view) and the
+    oracle."""
+    base, ks, cs = 2, [3, 5, 7, 11, 13, 15, 17, 19], [1, -1, 1, -1, 1, -1, 1, -1]
+    a = _make_abcd(tmp_path / "in.abcd", base, ks, cs, 20000, 60000, 0.6, 3)
+    p0, p1 = 10**9, 10**9 + 4 * 10**6
+    job = O.make_job(base, [int(x) for x in a.k], [int(x) for x in a.c], a.nmin, a.nmax)
+    cnt, facs, nf = O.sieve(job, p0, p1, a.bitmap().copy())
+    want = {(int(a.k[i]), int(a.c[i]), n) for (p, n, i) in facs}
+    got = {}
+    for flags, name in [([], "plain"), (["-l", "-b", "4"], "learned")]:
+        d = tmp_path / name
+        d.mkdir()
+        (d / "in.abcd").write_text((tmp_path / "in.abcd").read_text())
+        r = _run(["-p", str(p0), "-P", str(p1), "-i", "in.abcd", "-f", "out.txt"] + flags, str(d), timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        if flags:
+            assert "Cost model: candidate" in r.stdout and "Batched: 4 views per round" in r.stdout
+        terms = _terms(d / "out.txt", base)
+        for (p, k, c, n) in terms:
+            assert R.verify_factor(p, k, n, c, base) == 1
+        got[name] = {(k, c, n) for (p, k, c, n) in terms}
+        summary = [l for l in r.stdout.splitlines() if l.startswith("factors ")][-1]
+        assert ("prime count %d," % cnt.primecount) in summary
+        assert "Primes tested per second:" in r.stdout
+    assert got["plain"] == got["learned"] == want
+
+
+def test_cli_resume_mid_range(tmp_path):
+    """A run resumed from a checkpoint at mid-range (cl_sieve.cpp:797-863,
+    1047-1063): the factor file's lines clear their terms from the bitmap,
+    the rest of the range is sieved, and the file ends with the same term set
+    as an uninterrupted run, each term once.  A factor file with fewer lines
+    than the checkpoint counts stops with "Missing factors"."""
+    base, ks, cs = 2, [3, 5, 7, 11, 13, 15, 17, 19], [1, -1, 1, -1, 1, -1, 1, -1]
+    a = _make_abcd(tmp_path / "in.abcd", base, ks, cs, 20000, 60000, 0.6, 4)
+    p0, p1 = 10**8, 10**8 + 10**6
+    full = tmp_path / "full"
+    full.mkdir()
+    (full / "in.abcd").write_text((tmp_path / "in.abcd").read_text())
+    args = ["-p", str(p0), "-P", str(p1), "-i", "in.abcd", "-f", "out.txt"]
+    r = _run(args, str(full))
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = _factor_lines(full / "out.txt")
+    all_terms = {(k, c, n) for (p, k, c, n) in _terms(full / "out.txt", base)}
+    mid = p0 + (p1 - p0) // 2
+    before = [l for l in lines if int(l.split(" | ")[0]) < mid]
+    assert before and len(before) < len(lines)
+    job = O.make_job(base, [int(x) for x in a.k], [int(x) for x in a.c], a.nmin, a.nmax)
+    cnt_before, _, _ = O.sieve(job, p0, mid, a.bitmap().copy())
+    cnt_full, _, _ = O.sieve(job, p0, p1, a.bitmap().copy())
+
+    def checkpoint(d, nlines):
+        st = R.State()
+        st.pmin, st.pmax, st.p = p0, p1, mid
+        st.base, st.nmin, st.nmax, st.kcount = base, a.nmin, a.nmax, len(ks)
+        for i in range(len(ks)):
+            st.klist[i] = int(a.k[i]) if int(a.c[i]) > 0 else -int(a.k[i])
+        st.factorcount = len(before)
+        st.primecount = cnt_before.primecount
+        wa = C.c_int(1)
+        assert R.lib().rs_state_write(str(d).encode(), st, wa) == 0
+        (d / "in.abcd").write_text((tmp_path / "in.abcd").read_text())
+        (d / "out.txt").write_text("".join(l + "\n" for l in before[:nlines]))
+
+    import ctypes as C
+    res = tmp_path / "resumed"
+    res.mkdir()
+    checkpoint(res, len(before))
+    r = _run(args, str(res))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Resuming from checkpoint" in r.stdout and ("Updated bitmap with %d factors" % len(before)) in r.stdout
+    got = _terms(res / "out.txt", base)
+    keys = [(k, c, n) for (p, k, c, n) in got]
+    assert len(keys) == len(set(keys))
+    assert set(keys) == all_terms
+    summary = [l for l in r.stdout.splitlines() if l.startswith("factors ")][-1]
+    assert summary.startswith("factors %d, prime count %d," % (len(keys), cnt_full.primecount))
+
+    short = tmp_path / "short"
+    short.mkdir()
+    checkpoint(short, len(before) - 1)
+    r = _run(args, str(short))
+    assert r.returncode != 0 and "Missing factors" in r.stdout

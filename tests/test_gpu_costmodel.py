@@ -44,3 +44,32 @@ def test_cost_model_beats_reference_rule(ctx):
     # the untrained model falls back to the reference rule
     fresh = tuning.CostModel(ctx, Y.shape[1])
     assert np.all(fresh.choose(X[ntr:]) == 0)
+
+
+SPLIT_RECORDS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "split_records.npz")
+
+
+@pytest.mark.skipif(not os.path.isfile(SPLIT_RECORDS), reason="no recorded batch-split timings")
+def test_split_head_regret(ctx):
+    """The batch-split head (views per batched round and primes per segment,
+    rs_autotune_split timings of 384 synthetic jobs, written by
+    tests/golden/make_split_records.py): trained on the first 256 jobs, its
+    pick on the last 128 against the measured best.  It must beat one view per
+    round (no batch split) by far and stay within 1 % of the best on average;
+    the fixed splits are printed beside it (larger launches win for nearly
+    every job, so the largest option is close to the best as well)."""
+    import json
+    z = np.load(SPLIT_RECORDS, allow_pickle=False)
+    X, Y, V = z["X"], z["Y"], z["V"]
+    options = [tuple(o) for o in json.loads(str(z["options"]))]
+    ntr = 256
+    m = tuning.SplitModel(ctx, options, seed=3)
+    m.fit(X[:ntr], Y[:ntr], V[:ntr])
+    choice = m.choose(X[ntr:], V[ntr:])
+    r_model = tuning.regret(Y[ntr:], V[ntr:], choice)
+    fixed = {o: tuning.regret(Y[ntr:], V[ntr:], np.full(len(choice), i, np.int64)).mean()
+             for i, o in enumerate(options)}
+    print("held-out split regret: model mean %.4f | fixed %s" % (
+        r_model.mean(), ", ".join("%dx%d %.4f" % (v, sp, r) for (v, sp), r in fixed.items())), flush=True)
+    assert r_model.mean() < fixed[(1, 65536)] / 5
+    assert r_model.mean() < 0.01
