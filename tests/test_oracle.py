@@ -15,10 +15,19 @@ Parity status of each oracle function and the checks that pin it:
   or_getsegprimes/_in_range        Eratosthenes + the pseudoprime definition
   or_bsgs_params_for               invariants of the split (power of two,
                                    coverage M*Q >= L, LDS fit)
-  or_setup_prime                   brute-force discrete logs for small p
+  or_profile_range                 its defining equation x/ln x = 256*CU +
+                                   start/ln start, the 1.5x buffer, the cap,
+                                   and an actual prime count
+  or_setup_prime                   brute-force discrete logs for small primes;
+                                   for composite 2-PRPs (42799 = 127*337, ...)
+                                   the closed form of setup.cl's decisions in
+                                   plain modular arithmetic (skip test, power-
+                                   residue tests, parity classes, targets)
   or_sieve                         brute-force divisibility of every term for
                                    small configurations; duplicate count from
-                                   the brute-force hit count of each window
+                                   the brute-force hit count of each window;
+                                   ranges with composite 2-PRPs: the terms a
+                                   composite clears are the closed form's
   or_is_prime, or_verify_factor    reference verify_factor.c (oracle/_ref),
                                    Python MR, the reference's example lines
 """
@@ -391,3 +400,180 @@ def test_verify_matches_reference_build():
         assert L.or_verify_factor(p, k, n, c, b) == Rf.ref_verify_factor(p, k, n, c, b)
     for n in list(range(3, 5000, 2)) + A001262:
         assert L.or_is_prime(n) == Rf.ref_is_prime(n)
+
+
+# --- composite 2-PRPs ---------------------------------------------------------
+#
+# The generator emits base-2 strong pseudoprimes with no factor <= 113 (42799 =
+# 127*337 is the first), and the setup runs on them as on primes.  For an odd
+# modulus the Montgomery arithmetic is exact modular arithmetic, so setup.cl's
+# decisions (setup.cl:137-208) have a closed form in plain integers:
+#   skipped      iff (prod |k| * b^Q)^(p-1) != 1 (mod p), or every k is impossible
+#   tests        d in {3,4,5,7,8,9,11,13,17,19,23,29} with d | p-1 and b^((p-1)/d) == 1
+#   rg           b^((p-1)/2): +1, -1 or 0 (anything else)
+#   class of k   h = -c * k^-1 mod p: 0 if some test has h^((p-1)/d) != 1; with
+#                ra = h^((p-1)/2): rg == -1 -> 2 (ra == 1), 3 (ra == -1), else 0;
+#                rg == 1 and ra == -1 -> 0; otherwise 1
+# (Euler's criterion is not sound for a composite p: these decisions are the
+# reference's, not the truth, and the sieve must reproduce them.)
+
+
+def _composite_2prps(limit):
+    out = []
+    for n in range(115, limit, 2):
+        if any(n % q == 0 for q in (3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79,
+                                    83, 89, 97, 101, 103, 107, 109, 113)):
+            continue
+        d, s = n - 1, 0
+        while d % 2 == 0:
+            d //= 2
+            s += 1
+        x = pow(2, d, n)
+        ok = x in (1, n - 1)
+        for _ in range(s - 1):
+            if ok:
+                break
+            x = x * x % n
+            ok = x == n - 1
+        if ok and not py_is_prime(n):
+            out.append(n)
+    return out
+
+
+COMPOSITES = _composite_2prps(1000000)
+
+
+def _setup_closed_form(base, ks, cs, p, Q):
+    """None if the reference skips p, else {kidx: (hadj in Montgomery form, class)}."""
+    X = pow(base, Q, p)
+    for k in ks:
+        X = X * k % p
+    if pow(X, p - 1, p) != 1:
+        return None
+    pm = p - 1
+    tests = [pm // d for d in (3, 4, 5, 7, 8, 9, 11, 13, 17, 19, 23, 29) if pm % d == 0 and pow(base, pm // d, p) == 1]
+    g = pow(base, pm // 2, p)
+    rg = 1 if g == 1 else -1 if g == p - 1 else 0
+    out = {}
+    for i, (k, c) in enumerate(zip(ks, cs)):
+        h = (-c) * pow(k, -1, p) % p
+        if any(pow(h, e, p) != 1 for e in tests):
+            continue
+        ra = pow(h, pm // 2, p)
+        if rg == -1:
+            cls = 2 if ra == 1 else 3 if ra == p - 1 else 0
+        elif rg == 1 and ra == p - 1:
+            cls = 0
+        else:
+            cls = 1
+        if cls:
+            out[i] = (mont(h, p), cls)
+    return out or None
+
+
+def test_composite_2prps_are_what_the_generator_emits():
+    assert COMPOSITES[:4] == [42799, 49141, 88357, 90751]
+    emitted = set(int(x) for x in O.primes_in_range(115, 1000000))
+    assert set(COMPOSITES) <= emitted
+    assert all(py_is_prime(int(x)) for x in emitted - set(COMPOSITES))
+
+
+@pytest.mark.parametrize("base", [2, 3, 5, 6, 7, 10])
+def test_setup_composite_2prps_closed_form(base):
+    """or_setup_prime on composite 2-PRPs against the closed form above: the
+    skip decision, every k's class and its target, for k's coprime to p and k's
+    sharing a factor with p (the invertibility test must skip those)."""
+    rng = random.Random(100 + base)
+    checked = skipped = 0
+    classes = set()
+    for p in COMPOSITES:
+        facs = [q for q in range(127, int(math.isqrt(p)) + 1, 2) if p % q == 0]
+        for trial in range(6):
+            ks = []
+            while len(ks) < 8:
+                k = rng.randrange(3, 10**6)
+                if math.gcd(k, base) == 1 and k not in ks:
+                    ks.append(k)
+            if trial == 5:
+                ks[3] = facs[0] * rng.randrange(1, 50)  # not invertible mod p
+            cs = [1 if i % 2 else -1 for i in range(len(ks))]
+            job = O.make_job(base, ks, cs, 1000, 5000)
+            for Q in (1024, 4096):
+                pr, kd = O.setup_prime(job, p, Q)
+                want = _setup_closed_form(base, ks, cs, p, Q)
+                if want is None:
+                    assert pr is None or not kd, (p, ks, Q)
+                    skipped += 1
+                    continue
+                assert pr is not None
+                got = {kidx: (hadj, par) for hadj, par, kidx in kd}
+                assert got == want, (p, base, ks, Q)
+                checked += 1
+                classes |= {par for (_, par) in want.values()}
+    assert checked >= 20 and skipped >= 50
+    if base in (2, 6):  # b a quadratic non-residue mod some composites: the parity classes occur
+        assert classes == {1, 2, 3}
+
+
+@pytest.mark.parametrize("base,pmin,pmax", [(2, 42000, 50000), (3, 88000, 92000), (10, 42000, 43000)])
+def test_sieve_composite_2prps_brute_force(base, pmin, pmax):
+    """The sieve over ranges holding composite 2-PRPs: every claim's modulus
+    divides its term, and the terms cleared by a composite p are exactly the
+    ones its closed-form classes admit (the solutions of k*b^n + c == 0 mod p
+    whose k the reference's tests keep, of the allowed parity)."""
+    from util import bitmap_terms, full_bitmap
+    ks = [3, 5, 7, 11, 13, 17] if base % 2 == 0 else [2, 4, 8, 10, 14, 16]
+    ks = [k for k in ks if math.gcd(k, base) == 1]
+    cs = [1 if i % 2 else -1 for i in range(len(ks))]
+    nmin, nmax = 1000, 9000
+    bm, wps = full_bitmap(len(ks), nmin, nmax)
+    comps = [p for p in COMPOSITES if pmin <= p < pmax]
+    assert comps
+    for p in comps:
+        # the composite alone: its claims are the closed form's solutions
+        job = O.make_job(base, ks, cs, nmin, nmax)
+        out = bm.copy()
+        cnt, facs, nf = O.sieve(job, p, p + 1, out)
+        assert cnt.primecount == 1
+        prm = O.bsgs_params(nmin, nmax, len(ks))
+        want = _setup_closed_form(base, ks, cs, p, prm.Q)
+        sols = set()
+        for i, (hm, cls) in (want or {}).items():
+            for n in range(nmin, nmax + 1):
+                if (ks[i] * pow(base, n, p) + cs[i]) % p == 0 and (cls == 1 or n % 2 == cls - 2):
+                    sols.add((i, n))
+        assert {(i, n) for (_, n, i) in facs} == sols, (p, base)
+        for q, n, i in facs:
+            assert q == p and (ks[i] * pow(base, n, p) + cs[i]) % p == 0
+    # the whole range: every claim's modulus divides its term
+    job = O.make_job(base, ks, cs, nmin, nmax)
+    out = bm.copy()
+    cnt, facs, nf = O.sieve(job, pmin, pmax, out)
+    assert facs
+    for q, n, i in facs:
+        assert (ks[i] * pow(base, n, q) + cs[i]) % q == 0
+    assert bitmap_terms(bm, len(ks), wps, nmin) - bitmap_terms(out, len(ks), wps, nmin) == {(i, n) for (_, n, i) in facs}
+
+
+def test_profile_range_solves_its_equation():
+    """or_profile_range (cl_sieve.cpp:693-735): x = start + range solves
+    x / ln x = 256 * CU + start / ln start, so the segment holds about 256 * CU
+    primes by the prime number theorem; psize = 1.5 * 256 * CU; the range is
+    capped at 4294900000.  Checked against the equation and against an actual
+    prime count."""
+    import ctypes as C
+    L = O.lib()
+    for start in (10**9, 2**40, 2**50, 2**63 - 2**40):
+        for cu in (8, 104, 256):
+            r, ps = C.c_uint64(), C.c_uint32()
+            L.or_profile_range(start, cu, C.byref(r), C.byref(ps))
+            x = start + r.value
+            target = 256 * cu + start / math.log(start)
+            assert abs(x / math.log(x) - target) < 1e-6 * target + 2.0
+            assert ps.value == int(1.5 * 256 * cu)
+    r, ps = C.c_uint64(), C.c_uint32()
+    L.or_profile_range(10**9, 8, C.byref(r), C.byref(ps))
+    n = len(O.primes_in_range(10**9, 10**9 + r.value))
+    assert abs(n - 2048) < 0.1 * 2048
+    L.or_profile_range(10**18, 10**6, C.byref(r), C.byref(ps))
+    assert r.value == 4294900000
