@@ -6,6 +6,7 @@ numpy arrays across the C ABI.  Every compute call goes to the HIP kernels:
 there is no CPU fallback, and loading fails loudly when the library is
 missing.
 """
+import atexit
 import ctypes as C
 import os
 import weakref
@@ -302,6 +303,20 @@ def presieve(base, k, c, nmin, nmax, bitmap, words_per_seq, bound, nthreads=8):
 
 # --- device ---------------------------------------------------------------
 
+_live_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts():
+    # contexts (and their jobs and MLPs) still open at exit are closed while
+    # the interpreter and the HIP runtime are both intact
+    for ctx in list(_live_contexts):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+
+
 class Context:
     """A HIP device context (rs_open)."""
 
@@ -309,6 +324,7 @@ class Context:
         h = C.c_void_p()
         _check(lib().rs_open(device, C.byref(h)))
         self._h = h
+        _live_contexts.add(self)
         # jobs and MLPs live on the context's stream and device memory: they
         # are closed before it
         self._children = weakref.WeakSet()
@@ -340,7 +356,7 @@ class Context:
         n = C.c_int()
         _check(lib().rs_tune_candidates(self._h, nmin, nmax, kcount, arr, cap, C.byref(n)))
         if n.value > cap:
-            raise RSError("rs_tune_candidates: %d candidates, room for %d" % (n.value, cap))
+            raise RSError(RS_E_ARG, "rs_tune_candidates: %d candidates, room for %d" % (n.value, cap))
         return [arr[i] for i in range(n.value)]
 
     def mlp(self, n_in, n_hidden, n_out, seed=1):

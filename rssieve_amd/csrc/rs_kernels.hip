@@ -1199,7 +1199,8 @@ __device__ __forceinline__ uint32_t guide_start(const Guide& g, uint32_t k) {
 // 256-thread workgroups per CU.
 constexpr uint32_t kQCap = 384;
 constexpr uint32_t kHQ = 1024;  // filter words over the queue (qmix gives 10 index bits)
-constexpr uint32_t kHitCap = 128;
+constexpr uint32_t kHitCap = 64;
+constexpr uint32_t kWCap = 257;  // baby values of entries 64j: j <= ceil(16383 / 64)
 
 // One persistent kernel serves every job of a batch.  An item is (job, prime,
 // class) for a job with the reference's three tables (giant.cl giantfull /
@@ -1228,8 +1229,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
   __shared__ uint32_t s_item, s_job, s_order, s_dups, s_claims, s_qn;
   __shared__ uint32_t s_next;  // the block's next item when it is known (same chunk and job), else ~0
   __shared__ uint4 s_q[kQCap];  // filter positives of the item: residue, target, giant step
-  __shared__ uint32_t s_hq[kHQ];  // Bloom filter of the queued residues (match pass)
+  // Bloom filter of the queued residues (order path) or hash of the W_j (discrete logs)
+  __shared__ __align__(16) uint32_t s_hq[kHQ];
   __shared__ uint2 s_hit[kHitCap];  // baby-step hits of a match pass: (entry, queue slot)
+  __shared__ uint64_t s_W[kWCap];   // baby values of the entries 64j (the positives' discrete logs)
+  __shared__ uint64_t s_pf[64];     // b^(stride*f), f < 64 (the positives' discrete logs)
   __shared__ uint32_t s_nhit;
   const uint32_t tid = threadIdx.x;
   const uint32_t njobs = bs->njobs;
@@ -1289,8 +1293,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     // workgroup-uniform values read from LDS go through readfirstlane, so
     // the compiler keeps the item's values (and the loads that follow from
     // them) in scalar registers
+    // (thread 0 writes them again only after the item's last barrier; an
+    // item without targets passes one barrier of its own first)
     const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_item), jj = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_job);
-    __syncthreads();
     if (item >= nitems) break;
     const SegJob& J = jobs[jj];
     const JobDev* __restrict__ job = J.job;
@@ -1299,7 +1304,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     const uint32_t pi = merged ? local : local / 3, c = merged ? 0 : local % 3;
     const PrimeRec* __restrict__ R = J.recs + pi;
     const uint32_t nk = (uint32_t)__builtin_amdgcn_readfirstlane((int)R->nk[c]);
-    if (nk == 0) continue;
+    if (nk == 0) {
+      __syncthreads();
+      continue;
+    }
     const uint32_t hsize = job->hsize;
     const uint32_t base = job->base;
     const int K = job->kcount;
@@ -1466,6 +1474,80 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       }
     };
 
+    // Exact resolution of the queued positives of a table without the order
+    // shortcut, one discrete logarithm each.  With f = lane (0..63), a queued
+    // value g is the baby value of entry e = 64j - f exactly when
+    // g * b^(stride*f) = W_j, the baby value of entry 64j, for the one j in
+    // [0, J], J = ceil((E - 1) / 64).  The baby walk leaves W_j (64j < E) in
+    // s_W, and one product gives W_J when 64J >= E (whash_build, after the
+    // walk; complete at the first settle's barrier).  The hash s_hq holds a
+    // 23-bit fingerprint of each W_j and j + 1 in buckets of four slots
+    // (kHB buckets picked by residue bits 2..9, linear probing over buckets;
+    // a bucket fills its slots in order, so a search ends at the first bucket
+    // with an empty last slot), and a fingerprint hit is compared with s_W[j]
+    // in full, so a hit is exact.  Every wavefront takes every kW-th queued
+    // value, two at a time: per lane and value one product (by b^(stride*f)
+    // from s_pf) and, nearly always, one 16-byte LDS read.  A value whose low
+    // 32 bits are 0 is never a hit (the reference's table cannot hold it,
+    // giant.cl:44-46, 61-63).
+    constexpr uint32_t kHB = kHQ / 4;
+    const uint32_t nJ = nent > 1 ? (nent - 1 + 63) / 64 : 0;  // J
+    auto wfp = [](uint64_t w) { return (uint32_t)(w >> 20) & 0x7FFFFFu; };
+    auto whash_build = [&]() {
+      for (uint32_t j = tid; j <= nJ; j += T) {
+        uint64_t w;
+        if (nJ && j == nJ && 64 * nJ >= nent) {  // W_J past the table: W_(J-1) * b^(64*stride)
+          w = fp ? fp_to_u64(fp_mulmod((double)s_W[nJ - 1], __longlong_as_double(s_pw[0][6]), pd, pinv))
+                 : mmul(s_W[nJ - 1], s_pw[0][6], p, q);
+          s_W[nJ] = w;
+        } else {
+          w = s_W[j];
+        }
+        const uint32_t entry = (wfp(w) << 9) | (j + 1);
+        for (uint32_t bk = ((uint32_t)w >> 2) & (kHB - 1);; bk = (bk + 1) & (kHB - 1)) {
+          uint32_t k = 0;
+          while (k < 4 && atomicCAS(&s_hq[4 * bk + k], 0u, entry) != 0u) k++;
+          if (k < 4) break;
+        }
+      }
+    };
+    auto dlog = [&](uint32_t nq) {
+      const uint64_t plane = s_pf[lane];
+      // the hits among one bucket's slots; true when the bucket is full (the search goes on)
+      auto scan = [&](const uint4& en, uint32_t fg, uint64_t g, uint64_t gam, uint32_t sq) {
+        const uint32_t sl[4] = {en.x, en.y, en.z, en.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          if (sl[k] && (sl[k] >> 9) == fg) {
+            const uint32_t j = (sl[k] & 511u) - 1;
+            const uint32_t e = 64 * j - lane;
+            if (64 * j >= lane && e < nent && (uint32_t)g && s_W[j] == gam) claim_hit(e, sq, 0u);
+          }
+        }
+        return en.w != 0u;
+      };
+      auto look = [&](uint32_t sq, uint64_t g, uint64_t gam, uint4 en, uint32_t bk) {
+        const uint32_t fg = wfp(gam);
+        while (scan(en, fg, g, gam, sq)) {
+          bk = (bk + 1) & (kHB - 1);
+          en = ((const uint4*)s_hq)[bk];
+        }
+      };
+      auto lpow = [&](uint64_t g) {
+        return fp ? fp_to_u64(fp_mulmod((double)g, __longlong_as_double(plane), pd, pinv)) : mmul(g, plane, p, q);
+      };
+      for (uint32_t sq = wave; sq < nq; sq += 2 * kW) {
+        const bool two = sq + kW < nq;  // wavefront-uniform
+        const uint4 qa = s_q[sq], qb = s_q[two ? sq + kW : sq];
+        const uint64_t ga = ((uint64_t)qa.y << 32) | qa.x, gb = ((uint64_t)qb.y << 32) | qb.x;
+        const uint64_t gama = lpow(ga), gamb = lpow(gb);
+        const uint32_t ba = ((uint32_t)gama >> 2) & (kHB - 1), bb = ((uint32_t)gamb >> 2) & (kHB - 1);
+        const uint4 ea = ((const uint4*)s_hq)[ba], eb = ((const uint4*)s_hq)[bb];
+        look(sq, ga, gama, ea, ba);
+        if (two) look(sq + kW, gb, gamb, eb, bb);
+      }
+    };
+
     // The overflow recovery: the bucketed table of the baby values is built
     // in this workgroup's global scratch (at most half full) and every probe
     // of the tiles of steps ja on is looked up directly with integer
@@ -1513,11 +1595,13 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         redo(r0);
         return true;
       }
-      if (nq && (final || 2 * nq >= qcap)) {  // no positive, nothing to match
-        match(nq, nent, 0);
-        __syncthreads();
-        if (tid == 0) s_qn = 0;
-        __syncthreads();
+      if (nq && (final || 2 * nq >= qcap)) {  // no positive, nothing to resolve
+        dlog(nq);
+        if (!final) {  // after the final one the item's last barrier follows, and the next item's fetch clears s_qn
+          __syncthreads();
+          if (tid == 0) s_qn = 0;
+          __syncthreads();
+        }
         r0 = r1;
       }
       return false;
@@ -1526,24 +1610,43 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     {
       const uint4 z = make_uint4(0u, 0u, 0u, 0u);
       for (uint32_t s = tid; s < hsize / 4; s += T) lds4[s] = z;  // the filter
+      for (uint32_t s = tid; s < kHQ / 4; s += T) ((uint4*)s_hq)[s] = z;  // the discrete-log hash
     }
+    // targets: Montgomery residues, as binary64 values on the binary64 path
+    // (the giant steps' operands; exact below 2^51)
     for (uint32_t i = tid; i < nk; i += T) {
-      s_h[i] = kr[i].hadj;
+      const uint64_t h = kr[i].hadj;
+      s_h[i] = fp ? (uint64_t)__double_as_longlong((double)h) : h;
       s_k[i] = kr[i].kidx | ((merged ? kr[i].cls : 0u) << 8);
     }
     if (tid == 0) s_order = 0xFFFFFFFFu;
     __syncthreads();
+    // vstart = v0 * b^(stride*lane) * b^(stride*64*wave): the lane's factor
+    // as a product tree over its six bits (wavefront 0 keeps it in s_pf for
+    // the discrete logs), the wavefront's bits by uniform branches
     if (fp) {
-      double v = (double)v0;
+      double f[6];
 #pragma unroll
-      for (int i = 0; i < kLgT; i++)
-        if ((tid >> i) & 1u) v = fp_mulmod(v, __longlong_as_double(s_pw[0][i]), pd, pinv);
-      vstart = fp_to_u64(v);
+      for (int i = 0; i < 6; i++) f[i] = ((lane >> i) & 1u) ? __longlong_as_double(s_pw[0][i]) : 1.0;
+      const double f01 = fp_mulmod_signed(f[0], f[1], pd, pinv), f23 = fp_mulmod_signed(f[2], f[3], pd, pinv),
+                   f45 = fp_mulmod_signed(f[4], f[5], pd, pinv);
+      const double fl = fp_mulmod_signed(fp_mulmod_signed(f01, f23, pd, pinv), f45, pd, pinv);
+      if (tid < 64) s_pf[tid] = (uint64_t)__double_as_longlong(fl);
+      double v = fp_mulmod_signed((double)v0, fl, pd, pinv);
+#pragma unroll
+      for (int i = 6; i < kLgT; i++)
+        if ((wave >> (i - 6)) & 1u) v = fp_mulmod_signed(v, __longlong_as_double(s_pw[0][i]), pd, pinv);
+      vstart = fp_to_u64(fp_canon(v, pd));
     } else {
-      uint64_t v = v0;
+      uint64_t f[6];
 #pragma unroll
-      for (int i = 0; i < kLgT; i++)
-        if ((tid >> i) & 1u) v = mmul(v, s_pw[0][i], p, q);
+      for (int i = 0; i < 6; i++) f[i] = ((lane >> i) & 1u) ? s_pw[0][i] : one;
+      const uint64_t fl = mmul(mmul(mmul(f[0], f[1], p, q), mmul(f[2], f[3], p, q), p, q), mmul(f[4], f[5], p, q), p, q);
+      if (tid < 64) s_pf[tid] = fl;
+      uint64_t v = mmul(v0, fl, p, q);
+#pragma unroll
+      for (int i = 6; i < kLgT; i++)
+        if ((wave >> (i - 6)) & 1u) v = mmul(v, s_pw[0][i], p, q);
       vstart = v;
     }
     // Touch the next item's record and its first K target records' lines
@@ -1571,6 +1674,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         const uint32_t j = e * stride;
         if (j && v == v0m) atomicMin(&s_order, j);
         if ((uint32_t)v) flt_insert(flt, v);
+        if (lane == 0) s_W[e >> 6] = v;  // e = 64 * (e >> 6) on lane 0 (T is a multiple of 64)
       });
     }
     __syncthreads();
@@ -1581,7 +1685,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       // values b^(nstart + stride*e) with stride*e < order are distinct and
       // hold every reachable residue (giant.cl:121-134); the targets are
       // matched against them directly.
-      for (uint32_t i = tid; i < nk; i += T) s_q[i] = make_uint4((uint32_t)s_h[i], (uint32_t)(s_h[i] >> 32), i, 0u);
+      for (uint32_t i = tid; i < nk; i += T) {
+        const uint64_t h = fp ? (uint64_t)__longlong_as_double((long long)s_h[i]) : s_h[i];
+        s_q[i] = make_uint4((uint32_t)h, (uint32_t)(h >> 32), i, 0u);
+      }
       __syncthreads();
       match(nk, (order + stride - 1) / stride, order);
       work = order / stride;  // the baby steps up to the repeat (giant.cl:121-134); no giant products
@@ -1683,10 +1790,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
           if (settle(r0, j1, j1 == nsteps)) break;  // the overflow recovery finished the item
         }
       };
+      whash_build();
       if (fp) {
-        // binary64 targets: the Montgomery residues as doubles (exact below 2^51)
-        for (uint32_t i = tid; i < nk; i += T) s_h[i] = (uint64_t)__double_as_longlong((double)s_h[i]);
-        __syncthreads();
         GiantFp ar;
         ar.pd = pd;
         ar.pinv = pinv;
@@ -1716,7 +1821,6 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       s_acc[4 + c] += 1;
       if (order != 0xFFFFFFFFu) s_acc[7] += 1;
     }
-    __syncthreads();
   }
   if (tid == 0) acc_flush();
 }
