@@ -1231,8 +1231,16 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
   __shared__ uint4 s_q[kQCap];  // filter positives of the item: residue, target, giant step
   // Bloom filter of the queued residues (order path) or hash of the W_j (discrete logs)
   __shared__ __align__(16) uint32_t s_hq[kHQ];
-  __shared__ uint2 s_hit[kHitCap];  // baby-step hits of a match pass: (entry, queue slot)
-  __shared__ uint64_t s_W[kWCap];   // baby values of the entries 64j (the positives' discrete logs)
+  // G^f, f < 64 (the giant steps' first tiles), or, on the order path, the
+  // baby-step hits of a match pass: (entry, queue slot).  One array for both
+  // keeps the static LDS within 16 KiB, so two workgroups with 64 KiB filters
+  // share a CU.
+  static_assert(kHitCap * sizeof(uint2) == 64 * sizeof(uint64_t), "s_gl and s_hit share one array");
+  __shared__ uint64_t s_gl[64];
+  uint2* const s_hit = (uint2*)s_gl;
+  // baby values of the entries 64j (the positives' discrete logs): binary64
+  // keys (kFpBias + residue) on the binary64 path, Montgomery residues otherwise
+  __shared__ uint64_t s_W[kWCap];
   __shared__ uint64_t s_pf[64];     // b^(stride*f), f < 64 (the positives' discrete logs)
   __shared__ uint32_t s_nhit;
   const uint32_t tid = threadIdx.x;
@@ -1497,7 +1505,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       for (uint32_t j = tid; j <= nJ; j += T) {
         uint64_t w;
         if (nJ && j == nJ && 64 * nJ >= nent) {  // W_J past the table: W_(J-1) * b^(64*stride)
-          w = fp ? fp_to_u64(fp_mulmod((double)s_W[nJ - 1], __longlong_as_double(s_pw[0][6]), pd, pinv))
+          w = fp ? fp_mulmod_bits(__longlong_as_double((long long)s_W[nJ - 1]) - kFpTwo52,
+                                  __longlong_as_double(s_pw[0][6]), pd, pinv, pd + kFpTwo52)
                  : mmul(s_W[nJ - 1], s_pw[0][6], p, q);
           s_W[nJ] = w;
         } else {
@@ -1533,8 +1542,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
           en = ((const uint4*)s_hq)[bk];
         }
       };
-      auto lpow = [&](uint64_t g) {
-        return fp ? fp_to_u64(fp_mulmod((double)g, __longlong_as_double(plane), pd, pinv)) : mmul(g, plane, p, q);
+      auto lpow = [&](uint64_t g) {  // in the form s_W holds
+        return fp ? fp_mulmod_bits((double)g, __longlong_as_double(plane), pd, pinv, pd + kFpTwo52)
+                  : mmul(g, plane, p, q);
       };
       for (uint32_t sq = wave; sq < nq; sq += 2 * kW) {
         const bool two = sq + kW < nq;  // wavefront-uniform
@@ -1623,7 +1633,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     __syncthreads();
     // vstart = v0 * b^(stride*lane) * b^(stride*64*wave): the lane's factor
     // as a product tree over its six bits (wavefront 0 keeps it in s_pf for
-    // the discrete logs), the wavefront's bits by uniform branches
+    // the discrete logs), the wavefront's bits by uniform branches; wavefront
+    // 0 also writes G^lane to s_gl (the giant steps' first tiles)
     if (fp) {
       double f[6];
 #pragma unroll
@@ -1632,6 +1643,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
                    f45 = fp_mulmod_signed(f[4], f[5], pd, pinv);
       const double fl = fp_mulmod_signed(fp_mulmod_signed(f01, f23, pd, pinv), f45, pd, pinv);
       if (tid < 64) s_pf[tid] = (uint64_t)__double_as_longlong(fl);
+      if (wave == 0) {
+        double g[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) g[i] = ((lane >> i) & 1u) ? __longlong_as_double(s_pw[1][i]) : 1.0;
+        const double g01 = fp_mulmod_signed(g[0], g[1], pd, pinv), g23 = fp_mulmod_signed(g[2], g[3], pd, pinv),
+                     g45 = fp_mulmod_signed(g[4], g[5], pd, pinv);
+        const double gl = fp_mulmod_signed(fp_mulmod_signed(g01, g23, pd, pinv), g45, pd, pinv);
+        s_gl[lane] = (uint64_t)__double_as_longlong(fp_canon(gl, pd));
+      }
       double v = fp_mulmod_signed((double)v0, fl, pd, pinv);
 #pragma unroll
       for (int i = 6; i < kLgT; i++)
@@ -1643,6 +1663,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       for (int i = 0; i < 6; i++) f[i] = ((lane >> i) & 1u) ? s_pw[0][i] : one;
       const uint64_t fl = mmul(mmul(mmul(f[0], f[1], p, q), mmul(f[2], f[3], p, q), p, q), mmul(f[4], f[5], p, q), p, q);
       if (tid < 64) s_pf[tid] = fl;
+      if (wave == 0) {
+        uint64_t g[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) g[i] = ((lane >> i) & 1u) ? s_pw[1][i] : one;
+        s_gl[lane] = mmul(mmul(mmul(g[0], g[1], p, q), mmul(g[2], g[3], p, q), p, q), mmul(g[4], g[5], p, q), p, q);
+      }
       uint64_t v = mmul(v0, fl, p, q);
 #pragma unroll
       for (int i = 6; i < kLgT; i++)
@@ -1667,18 +1693,60 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         pf = *a;
       }
     }
-    // baby steps set the filter bits and find ord_p(b) below the span
-    {
-      const uint64_t v0m = v0;
+    // Baby steps: set the filter bits, keep W_j, the value of entry 64j
+    // (lane 0's values: entry e = 64*wave + T*i is W_(wave + kW*i)), and find
+    // ord_p(b) below the span (a repeat of v0).  A value whose low 32 bits
+    // are 0 is inserted like any other: its probes pass the filter and are
+    // rejected by the exact resolution, as the reference's table never holds
+    // such a value (giant.cl:44-46, 61-63).
+    if (fp) {
+      // binary64 keys (kFpBias + residue: the filter reads residue bits below
+      // 49 only), four chains of signed products stepping by b^(4*stride*T)
+      const uint64_t k0 = v0 ^ kFpBias;
+      const double pc = pd + kFpTwo52;
+      auto visit = [&](uint32_t e, uint64_t key) {
+        if (key == k0) atomicMin(&s_order, e * stride - 1u);  // entry 0 gives ~0: no repeat
+        flt_insert(flt, key);
+      };
+      double v[4];
+      v[0] = (double)vstart;
+#pragma unroll
+      for (int c = 1; c < 4; c++) v[c] = fp_mulmod_signed(v[c - 1], bsd, pd, pinv);
+      const double b2 = fp_mulmod_signed(bsd, bsd, pd, pinv), b4 = fp_mulmod_signed(b2, b2, pd, pinv);
+      uint32_t e = tid, jw = wave;
+      for (; e + 3 * T < nent; e += 4 * T, jw += 4 * kW) {
+        uint64_t key[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          key[c] = fp_signed_bits(v[c], pc);
+          visit(e + c * T, key[c]);
+        }
+        if (lane == 0) {
+#pragma unroll
+          for (int c = 0; c < 4; c++) s_W[jw + c * kW] = key[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) v[c] = fp_mulmod_signed(v[c], b4, pd, pinv);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        if (e + c * T < nent) {
+          const uint64_t key = fp_signed_bits(v[c], pc);
+          visit(e + c * T, key);
+          if (lane == 0) s_W[jw + c * kW] = key;
+        }
+      }
+    } else {
       baby_walk(nent, [&](uint32_t e, uint64_t v) {
-        const uint32_t j = e * stride;
-        if (j && v == v0m) atomicMin(&s_order, j);
-        if ((uint32_t)v) flt_insert(flt, v);
+        if (v == v0) atomicMin(&s_order, e * stride - 1u);
+        flt_insert(flt, v);
         if (lane == 0) s_W[e >> 6] = v;  // e = 64 * (e >> 6) on lane 0 (T is a multiple of 64)
       });
     }
     __syncthreads();
-    const uint32_t order = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_order);
+    // the first repeat's exponent offset (s_order holds it minus one), or ~0
+    const uint32_t omin = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_order);
+    const uint32_t order = omin == 0xFFFFFFFFu ? omin : omin + 1u;
     unsigned long long work, probes;
     if (order != 0xFFFFFFFFu) {
       // ord_p(b) (or its even multiple) is below the table span: the baby
@@ -1694,27 +1762,23 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       work = order / stride;  // the baby steps up to the repeat (giant.cl:121-134); no giant products
       probes = nk;
     } else {
-      // The wavefront's tiles, software-pipelined: tile j's products are
-      // computed while tile j-1's filter words arrive, then tile j-1 is
-      // tested and the next tile's targets and tile j's filter words are
-      // read.  Two register sets alternate (no copies, which would wait for
-      // the loads).  A tile is consumed before the next reads are issued, so
-      // a positive's returning LDS atomic waits only for reads issued half an
-      // iteration earlier.
+      // The wavefront's tiles, one at a time: products, filter words, test.
+      // The three workgroups a CU holds cover each other's LDS waits; a
+      // software pipeline over two tiles (products of tile j while tile j-1's
+      // words arrive) measured 5 % slower, the compiler copying its two
+      // register sets at every step.
       auto giant = [&](const auto& ar) __attribute__((always_inline)) {
         using A = typename std::decay<decltype(ar)>::type;
         using Tg = typename A::Tgt;
         using Pw = typename A::Pow;
-        // this wavefront's first tile and its giant power G^(64*qb + lane)
-        // (64*qb + lane < T: popcount products from the power table)
+        // this wavefront's first tile and its giant power G^(64*qb + lane):
+        // G^lane from s_gl times the power table's entries for the bits of
+        // the wavefront-uniform qb
         uint32_t qb = wave / ng, g = wave % ng;
-        Pw gq = ar.unit();
-        {
-          const uint32_t e0 = 64 * qb + lane;
+        Pw gq = ar.power(s_gl[lane]);
 #pragma unroll
-          for (int i = 0; i < kLgT; i++)
-            if ((e0 >> i) & 1u) gq = ar.mul(gq, ar.power(s_pw[1][i]));
-        }
+        for (int i = 6; i < kLgT; i++)
+          if ((qb >> (i - 6)) & 1u) gq = ar.mul(gq, ar.power(s_pw[1][i]));
         const Pw g64 = ar.power(s_pw[1][6]);  // G^64: one tile row further
         Tg tt[kU];
         auto load_t = [&]() {
@@ -1761,29 +1825,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
           const uint32_t tf = j0 * kW + wave;
           const uint32_t nt = tf < ntiles ? min(j1 - j0, (ntiles - tf + kW - 1) / kW) : 0u;
           if (nt) {
-            uint64_t uA[kU], uB[kU];
-            uint32_t wA[kU], wB[kU], iA, iB, qA, qB;
-            mul_t(uA, iA, qA);
-            load_t();
-            read_w(uA, wA);
-            uint32_t n = 1;
-            for (; n + 1 < nt; n += 2) {
-              mul_t(uB, iB, qB);
-              consume(uA, wA, iA, qA);
-              load_t();
-              read_w(uB, wB);
+            uint64_t uA[kU];
+            uint32_t wA[kU], iA, qA;
+            for (uint32_t n = 0; n < nt; n++) {
               mul_t(uA, iA, qA);
-              consume(uB, wB, iB, qB);
               load_t();
               read_w(uA, wA);
-            }
-            if (n < nt) {
-              mul_t(uB, iB, qB);
-              consume(uA, wA, iA, qA);
-              load_t();
-              read_w(uB, wB);
-              consume(uB, wB, iB, qB);
-            } else {
               consume(uA, wA, iA, qA);
             }
           }

@@ -209,6 +209,9 @@ RS_HD uint64_t fp_mulmod_bits(double a, double b, double p, double pinv, double 
   return RS_AS_U64(RS_FADD(t, t < 0.0 ? pc : kFpTwo52));
 }
 
+// kFpBias + (t mod p) for t in (-p, p); pc = p + 2^52
+RS_HD uint64_t fp_signed_bits(double t, double pc) { return RS_AS_U64(RS_FADD(t, t < 0.0 ? pc : kFpTwo52)); }
+
 // integer residue of a canonical double
 RS_HD uint64_t fp_to_u64(double c) { return RS_AS_U64(RS_FADD(c, kFpTwo52)) ^ kFpBias; }
 
