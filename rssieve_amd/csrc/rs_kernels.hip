@@ -14,9 +14,10 @@
 //               compaction (the reference's sort.cl, folded in)
 //  round_items  prefix sums of the work items of every job
 //  bsgs         persistent workgroups pull (job, prime, class) items from one
-//               device queue; the baby-step table lives entirely in LDS
-//               (32-bit fingerprint + 16-bit index per slot, no global spill)
-//               and giant steps probe it (giant.cl:77-323)
+//               device queue; the baby steps fill a blocked Bloom filter in
+//               LDS, the giant steps probe it, and the rare positives are
+//               resolved exactly by a discrete logarithm against every 64th
+//               baby value (giant.cl:77-323)
 //
 // Because the cursors live on the device, the round can be captured once in
 // a HIP graph and replayed until every job's range is done.
@@ -28,6 +29,26 @@
 #include "rs_mont.h"
 
 namespace rs {
+
+// Loads of data a kernel only reads (written by earlier launches): through
+// the constant address space, so that a wavefront-uniform address is one
+// scalar load, or the global address space for per-lane addresses (a generic
+// pointer would make a flat load that waits on both counters).  Scalar
+// types only.
+template <class T>
+__device__ __forceinline__ T ld_const(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)(uintptr_t)p;
+}
+template <class T>
+__device__ __forceinline__ T ld_global(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)(uintptr_t)p;
+}
+__device__ __forceinline__ uint32_t rfl(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+template <class T>
+__device__ __forceinline__ T* rflp(T* p) {
+  const uint64_t a = (uint64_t)(uintptr_t)p;
+  return (T*)(uintptr_t)(((uint64_t)rfl((uint32_t)(a >> 32)) << 32) | rfl((uint32_t)a));
+}
 
 // ---------------------------------------------------------------------------
 // Small-prime bit tables: bit i of table_p[r] is set iff p divides r + 2i,
@@ -1226,8 +1247,22 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
   __shared__ uint64_t s_h[2 * RS_MAXK + 16];  // targets of the item (+ read-ahead pad)
   static_assert(RS_MAXK <= 256, "a sequence index fits the low 8 bits of s_k");
   __shared__ uint32_t s_k[2 * RS_MAXK];   // sequence index | shift (< 2^21) << 8
-  __shared__ uint32_t s_item, s_job, s_order, s_dups, s_claims, s_qn;
+  __shared__ uint32_t s_item, s_order, s_dups, s_claims, s_qn;
   __shared__ uint32_t s_next;  // the block's next item when it is known (same chunk and job), else ~0
+  // The fields of the block's current job that every item reads, copied by
+  // thread 0 when the job changes (its items are read through these, not
+  // through the job's descriptors, which would cost a chain of dependent
+  // global loads per item).
+  struct JobCache {
+    const JobDev* job;
+    const PrimeRec* recs;
+    const KRec* krecs;
+    uint32_t* bitmap;
+    FactorRec* fac;
+    Counters* cnt;
+    uint32_t item_base, algo, hsize, base, kcount, nmin, nmax, Q, QQ, M, MM, qcap;
+  };
+  __shared__ JobCache s_jc;
   __shared__ uint4 s_q[kQCap];  // filter positives of the item: residue, target, giant step
   // Bloom filter of the queued residues (order path) or hash of the W_j (discrete logs)
   __shared__ __align__(16) uint32_t s_hq[kHQ];
@@ -1253,6 +1288,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
   const Guide gd = guide_make(nitems, gridDim.x);
   uint32_t tk = 0, ci = 0, ce = 0;       // ticket in flight, current chunk [ci, ce)
   uint32_t jcur = 0, jend = 0;           // jend: first item past job jcur
+  uint32_t jcached = 0xFFFFFFFFu;        // the job in s_jc
   // Thread 0 sums the counters of its items in LDS and adds them to the
   // job's once (when the block moves to the next job and at exit): every
   // block adding per item would serialise on the few counter addresses.
@@ -1288,10 +1324,33 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         tk = atomicAdd(&bs->work_next, 1u);
       }
       const uint32_t it = ci < ce ? ci++ : nitems;
-      if (it < nitems)
+      if (it < nitems) {
         while (it >= jend) jend = bs->item_base[++jcur + 1];
+        if (jcur != jcached) {
+          const SegJob& J = jobs[jcur];
+          const JobDev* jd = J.job;
+          s_jc.job = jd;
+          s_jc.recs = J.recs;
+          s_jc.krecs = J.krecs;
+          s_jc.bitmap = J.bitmap;
+          s_jc.fac = J.fac;
+          s_jc.cnt = J.cnt;
+          s_jc.item_base = bs->item_base[jcur];
+          s_jc.algo = jd->algo;
+          s_jc.hsize = jd->hsize;
+          s_jc.base = jd->base;
+          s_jc.kcount = (uint32_t)jd->kcount;
+          s_jc.nmin = jd->nmin;
+          s_jc.nmax = jd->nmax;
+          s_jc.Q = jd->Q;
+          s_jc.QQ = jd->QQ;
+          s_jc.M = jd->M;
+          s_jc.MM = jd->MM;
+          s_jc.qcap = jd->qcap;
+          jcached = jcur;
+        }
+      }
       s_item = it;
-      s_job = jcur;
       s_next = (ci < ce && ci < jend) ? ci : 0xFFFFFFFFu;
       s_dups = 0;
       s_claims = 0;
@@ -1303,43 +1362,53 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     // them) in scalar registers
     // (thread 0 writes them again only after the item's last barrier; an
     // item without targets passes one barrier of its own first)
-    const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_item), jj = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_job);
+    const uint32_t item = rfl(s_item);
     if (item >= nitems) break;
-    const SegJob& J = jobs[jj];
-    const JobDev* __restrict__ job = J.job;
-    const bool merged = job->algo != 0;  // one list per prime (merged or residue-class table)
-    const uint32_t local = item - bs->item_base[jj];
+    const JobDev* __restrict__ job = rflp(s_jc.job);
+    const uint32_t jalgo = rfl(s_jc.algo);
+    const bool merged = jalgo != 0;  // one list per prime (merged or residue-class table)
+    const uint32_t local = item - rfl(s_jc.item_base);
     const uint32_t pi = merged ? local : local / 3, c = merged ? 0 : local % 3;
-    const PrimeRec* __restrict__ R = J.recs + pi;
-    const uint32_t nk = (uint32_t)__builtin_amdgcn_readfirstlane((int)R->nk[c]);
+    const PrimeRec* __restrict__ R = rflp(s_jc.recs) + pi;
+    // the record's fields in one batch of scalar loads, before the test of nk
+    const uint32_t nk = ld_const(&R->nk[c]), nk0 = ld_const(&R->nk[0]), nk1 = ld_const(&R->nk[1]);
+    const uint64_t p = ld_const(&R->p), q = ld_const(&R->q), one = ld_const(&R->one), mbase = ld_const(&R->mbase);
+    const uint64_t rbn = ld_const(&R->bn), rbT0 = ld_const(&R->bT[0]), rbT1 = ld_const(&R->bT[1]);
+    const uint64_t rg0 = ld_const(&R->g[0]), rg1 = ld_const(&R->g[1]);
+    const uint32_t rS = ld_const(&R->S), rE = ld_const(&R->E), rM = ld_const(&R->M);
+    // (an empty asm that reads them all keeps the loads together, ahead of the branch)
+    asm volatile("" ::"s"(nk), "s"(nk0), "s"(nk1), "s"(p), "s"(q), "s"(one), "s"(mbase), "s"(rbn), "s"(rbT0),
+                 "s"(rbT1), "s"(rg0), "s"(rg1), "s"(rS), "s"(rE), "s"(rM));
     if (nk == 0) {
       __syncthreads();
       continue;
     }
-    const uint32_t hsize = job->hsize;
-    const uint32_t base = job->base;
-    const int K = job->kcount;
-    const uint32_t nmin = job->nmin, nmax = job->nmax;
+    const uint32_t hsize = rfl(s_jc.hsize);
+    const uint32_t base = rfl(s_jc.base);
+    const int K = (int)rfl(s_jc.kcount);
+    const uint32_t nmin = rfl(s_jc.nmin), nmax = rfl(s_jc.nmax);
     const uint32_t cp = (c && !merged) ? 1 : 0;
-    const uint64_t p = R->p, q = R->q, one = R->one, mbase = R->mbase;
     const Filter flt = filter_make(tab, hsize, p);
     // table stride, baby-step entries, exponent span and giant steps of the
     // item: the reference's (three lists), fixed (merged), or per prime
     // (residue classes)
-    const bool rcls = job->algo == 2;
-    const uint32_t stride = rcls ? R->S : (cp || merged) ? 2 : 1;
-    const uint32_t nent = rcls ? R->E : (cp ? job->QQ : job->Q) / stride;
-    const uint32_t Ql = rcls ? stride * nent : cp ? job->QQ : job->Q;
-    const uint32_t Ml = (uint32_t)__builtin_amdgcn_readfirstlane((int)(rcls ? R->M : cp ? job->MM : job->M));
+    const bool rcls = jalgo == 2;
+    const uint32_t jQ = rfl(s_jc.Q), jQQ = rfl(s_jc.QQ), jM = rfl(s_jc.M), jMM = rfl(s_jc.MM);
+    const uint32_t stride = rcls ? rS : (cp || merged) ? 2 : 1;
+    const uint32_t nent = rcls ? rE : (cp ? jQQ : jQ) / stride;
+    const uint32_t Ql = rcls ? stride * nent : cp ? jQQ : jQ;
+    const uint32_t Ml = rcls ? rM : cp ? jMM : jM;
     const uint32_t nstart = nmin + (c == 2 ? 1 : 0);
-    const uint64_t v0 = (c == 2) ? mul_base(R->bn, mbase, base, p, q) : R->bn;
-    const uint64_t bstep = R->bT[cp];
-    const uint32_t koff = (c >= 1 ? R->nk[0] : 0) + (c >= 2 ? R->nk[1] : 0);
-    const KRec* __restrict__ kr = J.krecs + (size_t)pi * (merged ? 2 * K : K) + koff;
-    uint32_t* __restrict__ bitmap = J.bitmap;
-    FactorRec* __restrict__ fac = J.fac;
-    Counters* __restrict__ cnt = J.cnt;
-    const uint32_t qcap = job->qcap;
+    const uint64_t v0 = (c == 2) ? mul_base(rbn, mbase, base, p, q) : rbn;
+    const uint64_t bstep = cp ? rbT1 : rbT0;
+    const uint32_t koff = (c >= 1 ? nk0 : 0) + (c >= 2 ? nk1 : 0);
+    const KRec* __restrict__ jkrecs = rflp(s_jc.krecs);
+    const uint32_t kstride = merged ? 2 * K : K;  // target records per prime
+    const KRec* __restrict__ kr = jkrecs + (size_t)pi * kstride + koff;
+    uint32_t* __restrict__ bitmap = rflp(s_jc.bitmap);
+    FactorRec* __restrict__ fac = rflp(s_jc.fac);
+    Counters* __restrict__ cnt = rflp(s_jc.cnt);
+    const uint32_t qcap = rfl(s_jc.qcap);
     // Queue a filter positive.  Past qcap the entry is dropped and the
     // item's remaining rounds are redone exactly (settle); the hot loop
     // itself holds no table lookups, calls or global accesses.
@@ -1377,16 +1446,16 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     // 2*log2(T) threads before the barrier below.  A thread's first baby
     // value b^(nstart + stride*tid) and giant power G^tid then cost
     // popcount(tid) products instead of an exponentiation each.
-    const uint64_t Gm = R->g[cp];
+    const uint64_t Gm = cp ? rg1 : rg0;
     if (tid < 2 * kLgT) {
       const uint32_t i = tid % kLgT, side = tid / kLgT;
       if (fp) {
         double x = side ? (double)mmul(Gm, 1, p, q)
-                        : rcls ? (double)mmul(R->bT[1], 1, p, q) : fp_powmod_sm(bd, stride, pd, pinv);
+                        : rcls ? (double)mmul(rbT1, 1, p, q) : fp_powmod_sm(bd, stride, pd, pinv);
         for (uint32_t k = 0; k < i; k++) x = fp_mulmod(x, x, pd, pinv);
         s_pw[side][i] = (uint64_t)__double_as_longlong(x);
       } else {
-        uint64_t x = side ? Gm : rcls ? R->bT[1] : basepow_sm(mbase, stride, base, p, q, one);
+        uint64_t x = side ? Gm : rcls ? rbT1 : basepow_sm(mbase, stride, base, p, q, one);
         for (uint32_t k = 0; k < i; k++) x = mmul(x, x, p, q);
         s_pw[side][i] = x;
       }
@@ -1586,7 +1655,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
                 &s_dups, &s_claims);
         }
       };
-      const uint64_t G = R->g[cp];
+      const uint64_t G = Gm;
       for (uint32_t t = ja * kW + wave; t < ntiles; t += kW) {
         const uint32_t qi = (t / ng) * 64 + lane, g0 = (t % ng) * kU;
         if (qi >= Ml) continue;
@@ -1625,9 +1694,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     // targets: Montgomery residues, as binary64 values on the binary64 path
     // (the giant steps' operands; exact below 2^51)
     for (uint32_t i = tid; i < nk; i += T) {
-      const uint64_t h = kr[i].hadj;
+      const uint64_t h = ld_global(&kr[i].hadj);
+      const uint64_t kc = ld_global((const uint64_t*)&kr[i].kidx);  // kidx, cls
       s_h[i] = fp ? (uint64_t)__double_as_longlong((double)h) : h;
-      s_k[i] = kr[i].kidx | ((merged ? kr[i].cls : 0u) << 8);
+      s_k[i] = (uint32_t)kc | ((merged ? (uint32_t)(kc >> 32) : 0u) << 8);
     }
     if (tid == 0) s_order = 0xFFFFFFFFu;
     __syncthreads();
@@ -1685,12 +1755,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       // the targets of a prime rarely exceed K (room is 2K): K records' lines
       const uint32_t kbytes = (uint32_t)K * (uint32_t)sizeof(KRec);
       if (nx != 0xFFFFFFFFu && (tid == 0 || 128u * (tid - 1) < kbytes)) {
-        const uint32_t lx = nx - bs->item_base[jj];
+        const uint32_t lx = nx - rfl(s_jc.item_base);
         const uint32_t px = merged ? lx : lx / 3;
-        const uint32_t* a = tid == 0 ? (const uint32_t*)(J.recs + px)
-                                     : (const uint32_t*)((const char*)(J.krecs + (size_t)px * (merged ? 2 * K : K)) +
-                                                         128u * (tid - 1));
-        pf = *a;
+        const uint32_t* a = tid == 0 ? (const uint32_t*)(R - pi + px)
+                                     : (const uint32_t*)((const char*)(jkrecs + (size_t)px * kstride) + 128u * (tid - 1));
+        pf = ld_global(a);
       }
     }
     // Baby steps: set the filter bits, keep W_j, the value of entry 64j
