@@ -132,20 +132,21 @@ def copy_calibration(nbytes):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bsgs_cfg2_traffic.txt")
 
 
-def measured_traffic(path, kernel):
-    """HBM bytes per launch of the BSGS kernel on this path from the committed
-    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or
-    None."""
+def measured_traffic(path, kernel, primes_per_launch):
+    """HBM bytes per launch of the BSGS kernel on this path: the bytes per
+    prime of the committed PMC summary (FETCH_SIZE + WRITE_SIZE per launch,
+    scaled by a copy of known size, over the primes of that launch) times this
+    run's primes per launch, or None."""
     try:
         for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch %s %s:" % (path, kernel)):
-                return float(line.split(":")[1])
+            if line.startswith("traffic_bytes_per_prime %s %s:" % (path, kernel)):
+                return float(line.split(":")[1]) * primes_per_launch
     except OSError:
         return None
     return None
 
 
-def roofline(work_mmul, bsgs_s, launches, kernel, path):
+def roofline(work_mmul, bsgs_s, launches, kernel, path, primes):
     """The BSGS kernel against its compute roofline: algorithmic modular
     products (the kernel's own count: baby steps, giant-step products,
     giant-step advances) x 10 FP64 flops, per average launch duration (HIP
@@ -156,13 +157,14 @@ def roofline(work_mmul, bsgs_s, launches, kernel, path):
     per_launch = work_mmul / max(launches, 1)
     achieved = per_launch * FLOPS_PER_MODMUL / max(avg, 1e-12) / 1e12
     return {"bound": "valu", "compute": "fp64 VALU", "achieved": achieved, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / (FP64_PEAK / 1e12), "traffic": measured_traffic(path, kernel), "kernel": kernel,
+            "frac": achieved / (FP64_PEAK / 1e12),
+            "traffic": measured_traffic(path, kernel, primes / max(launches, 1)), "kernel": kernel,
             "launches": launches,
             "avg_launch_ms": avg * 1e3, "modmul_per_launch": per_launch,
             "modmul_per_s": per_launch / max(avg, 1e-12)}
 
 
-def roofline_int(work_mmul, bsgs_s, launches, kernel, peak_mad):
+def roofline_int(work_mmul, bsgs_s, launches, kernel, peak_mad, path, primes):
     """The BSGS kernel on the integer Montgomery path (p >= 2^51): its
     32x32-bit multiply instructions (14 per algorithmic product) per second
     against the measured v_mad_u64_u32 issue rate of the device."""
@@ -171,7 +173,8 @@ def roofline_int(work_mmul, bsgs_s, launches, kernel, peak_mad):
     achieved = per_launch * INT_MULS_PER_MODMUL / max(avg, 1e-12)
     return {"bound": "valu", "compute": "int32 multiply (v_mad_u64_u32 / v_mul_lo_u32 / v_mul_hi_u32)",
             "achieved": achieved / 1e12, "peak": peak_mad / 1e12, "unit": "Tmul/s", "frac": achieved / peak_mad,
-            "traffic": None, "kernel": kernel, "launches": launches, "avg_launch_ms": avg * 1e3,
+            "traffic": measured_traffic(path, kernel, primes / max(launches, 1)), "kernel": kernel, "launches": launches,
+            "avg_launch_ms": avg * 1e3,
             "modmul_per_launch": per_launch, "peak_source": "rs_peak_mad_u64_u32 on this device"}
 
 
@@ -215,7 +218,8 @@ def run_plain(ctx, data, steps, warmup):
     job.close()
     M, QQ, MM = R.derive(prm, nmin, nmax)
     return {"primes": c1.primecount - c0.primecount, "seconds": t1 - t0, "bitmap": bm,
-            "roof": roofline(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % prm.threads, "plain"),
+            "roof": roofline(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % prm.threads, "plain",
+                             c1.primecount - c0.primecount),
             "busy": bsgs_s / (t1 - t0), "claims": c1.claims, "dups": c1.dups, "verified": (nf, ok, prp),
             "cfg": {"algo": "three lists", "Q": prm.Q, "M": M, "QQ": QQ, "MM": MM, "hsize": prm.hsize,
                     "threads": prm.threads, "seg_primes": prm.seg_primes}}
@@ -436,7 +440,8 @@ def run_tuned(ctx, data, steps, warmup, batch):
         v.close()
     job.close()
     return {"primes": c1.primecount - c0.primecount, "seconds": t1 - t0, "bitmap": bm,
-            "roof": roofline(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % cfg.threads, "tuned"),
+            "roof": roofline(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % cfg.threads, "tuned",
+                             c1.primecount - c0.primecount),
             "busy": bsgs_s / (t1 - t0), "claims": c1.claims, "dups": c1.dups, "verified": (nf, ok, prp),
             "cfg": {"algo": ALGO_NAMES[cfg.algo], "Q": cfg.Q,
                     "hsize": cfg.hsize, "threads": cfg.threads, "seg_primes": cfg.seg_primes, "batch": batch,
@@ -482,7 +487,7 @@ def run_high_p(ctx, peak_mad, width=2**28):
         nf, ok, prp = drain_verified(job, base, 1 << 16)
         out[label] = {"primes": c1.primecount - c0.primecount, "seconds": dt, "bitmap": job.bitmap(),
                       "roof": roofline_int(c1.work_mmul - c0.work_mmul, bsgs_s, launches, "k_bsgs<%d>" % prm.threads,
-                                           peak_mad),
+                                           peak_mad, "high_p_" + label, c1.primecount - c0.primecount),
                       "busy": bsgs_s / dt, "verified": ok, "algo": ALGO_NAMES[prm.algo], "Q": prm.Q,
                       "threads": prm.threads, "views": nv, "seg_primes": prm.seg_primes}
         for v in views:
@@ -494,7 +499,7 @@ def run_high_p(ctx, peak_mad, width=2**28):
             "plain_primes_per_s": pl["primes"] / pl["seconds"], "tuned_primes_per_s": tu["primes"] / tu["seconds"],
             "tuned_over_plain": pl["seconds"] / tu["seconds"] * tu["primes"] / max(pl["primes"], 1),
             "same_terms": same, "tuned_config": {k: tu[k] for k in ("algo", "Q", "threads", "views", "seg_primes")},
-            "candidate": idx, "factors_verified": pl["verified"],
+            "candidate": idx, "factors_verified": pl["verified"], "primes": tu["primes"],
             "roofline": dict(tu["roof"], bsgs_busy_frac=tu["busy"]),
             "plain_roofline": dict(pl["roof"], bsgs_busy_frac=pl["busy"])}
 
@@ -513,6 +518,8 @@ def main():
     ap.add_argument("--no-jobs16", action="store_true", help="skip the 16-job batch leg")
     ap.add_argument("--other-workloads", action="store_true",
                     help="also time the small case and the high-p case (plain vs cost-model pick)")
+    ap.add_argument("--high-p-only", action="store_true",
+                    help="run only the high-p leg and print its record (for profiling its kernels)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a kernel of known traffic for PMC scaling)")
     args = ap.parse_args()
@@ -531,6 +538,11 @@ def main():
     ctx = R.Context(0)
     if args.copy_calibration_mib:
         copy_calibration(args.copy_calibration_mib << 20)
+    if args.high_p_only:
+        high_p = run_high_p(ctx, ctx.peak_mad_u64_u32())
+        ctx.close()
+        print(json.dumps({"high_p": high_p}))
+        return
     plain = None if args.tuned_only else run_plain(ctx, data, args.steps, args.warmup)
     res = plain
     tuned = None
