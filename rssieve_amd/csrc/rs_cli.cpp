@@ -16,6 +16,7 @@
 
 #include <unistd.h>
 
+#include "rs_boinc.h"
 #include "rssieve.h"
 #include "rssieve_io.h"
 #include "rssieve_mlp.h"
@@ -27,6 +28,9 @@ struct Options {
   std::string input, factor_file;
   bool selftest = false;
   int device = 0;
+  bool device_set = false;  // -d given (else init_data.xml's device, else 0)
+  std::string frac_file;    // -F: fraction done for a BOINC wrapper
+  std::string trickle_file; // -T: daily trickle-up message for a BOINC wrapper
   bool autotune = false;  // -t: time every candidate configuration on a sample, sieve with the fastest
   bool learned = false;   // -l / -m: the cost model picks the configuration and the batch split
   std::string model_path; // -m: model file (default: model/cost_model.bin beside the executable)
@@ -48,6 +52,9 @@ void help() {
   printf("-m modelfile\t\tLike -l with the given cost model file\n");
   printf("-b #\t\t\tSieve as # views of the range per batched round (one persistent launch for all)\n");
   printf("-c\t\t\tEncode k as the reference's signed int (k >= 2^31 wraps)\n");
+  printf("-F file\t\t\tWrite the fraction done to file every 2 seconds (a BOINC wrapper's fraction_done_filename)\n");
+  printf("-T file\t\t\tWrite the daily trickle-up message to file (a BOINC wrapper's trickle_up_filename)\n");
+  printf("\t\t\tWithout -d the device is init_data.xml's <gpu_device_num> when that file is present\n");
   printf("-s\t\t\tRun the built-in self test\n");
   printf("-h\t\t\tPrint this help\n");
   exit(EXIT_FAILURE);
@@ -110,7 +117,10 @@ void parse_cmdline(int argc, char** argv, Options* o) {
       if (tok) o->factor_file = tok;
     } else if (!strcmp(tok, "-d")) {
       tok = strtok_r(nullptr, " \t", &save);
-      if (tok) o->device = atoi(tok);
+      if (tok) {
+        o->device = atoi(tok);
+        o->device_set = true;
+      }
     } else if (!strcmp(tok, "-s")) {
       o->selftest = true;
     } else if (!strcmp(tok, "-t")) {
@@ -133,6 +143,12 @@ void parse_cmdline(int argc, char** argv, Options* o) {
       }
     } else if (!strcmp(tok, "-c")) {
       o->compat = true;
+    } else if (!strcmp(tok, "-F")) {
+      tok = strtok_r(nullptr, " \t", &save);
+      if (tok) o->frac_file = tok;
+    } else if (!strcmp(tok, "-T")) {
+      tok = strtok_r(nullptr, " \t", &save);
+      if (tok) o->trickle_file = tok;
     } else if (!strcmp(tok, "-h")) {
       help();
     }
@@ -230,6 +246,7 @@ struct Run {
   uint64_t factor_cap = 0;
   uint64_t base_primecount = 0, base_dups = 0;  // totals restored from a checkpoint
   std::vector<rs_factor> fbuf;
+  std::chrono::steady_clock::time_point t_start = std::chrono::steady_clock::now();  // this run's start
 };
 
 // cl_sieve.cpp:530-643: drain, sort by (p, n), verify on the CPU, append
@@ -295,7 +312,16 @@ void get_results(Run& r, bool print) {
     fprintf(stderr, "Verified %" PRIu64 " factors.\n", n - prp);
     if (print) printf("Verified %" PRIu64 " factors.\n", n - prp);
   }
-  // checkpoint (cl_sieve.cpp:321-328)
+  // checkpoint (cl_sieve.cpp:321-328), with the daily trickle-up message
+  // (cl_sieve.cpp:40-62) when -T names its file: progress, this process's
+  // CPU time and this run's elapsed time
+  if (!r.o.trickle_file.empty() && rs_boinc::trickle_due((uint64_t)time(nullptr), &r.st.last_trickle)) {
+    const double run = std::chrono::duration<double>(std::chrono::steady_clock::now() - r.t_start).count();
+    const std::string msg = rs_boinc::trickle_message(rs_boinc::fraction_done(r.st.pmin, r.st.pmax, r.st.p),
+                                                      (double)clock() / CLOCKS_PER_SEC, run);
+    if (!rs_boinc::write_file_atomic(r.o.trickle_file, msg))
+      fprintf(stderr, "Cannot write %s\n", r.o.trickle_file.c_str());
+  }
   if (rs_state_write(".", &r.st, &r.write_a_next)) fprintf(stderr, "%s\n", rs_last_error());
   if (print) printf("Checkpoint, current p: %" PRIu64 "\n", r.st.p);
 }
@@ -413,7 +439,23 @@ int sieve(Options o) {
   printf("Starting sieve at p: %" PRIu64 " n: %u\nStopping sieve at P: %" PRIu64 " N: %u\n", st.pmin, st.nmin,
          st.pmax, st.nmax);
 
-  if (rs_open(o.device, &r.ctx)) die_lib("rs_open");
+  // the device: -d, else the BOINC slot's init_data.xml (main.cpp:207-229)
+  int device = o.device;
+  if (!o.device_set) {
+    int d = 0;
+    const int got = rs_boinc::device_from_init_data("init_data.xml", &d);
+    if (got > 0) {
+      device = d;
+      printf("Using device %d from init_data.xml\n", device);
+    } else if (got < 0) {
+      fprintf(stderr, "Error: init_data.xml holds no valid gpu_device_num\n");
+      printf("Error: init_data.xml holds no valid gpu_device_num\n");
+      exit(EXIT_FAILURE);
+    } else {
+      printf("init_data.xml not found, using device 0.\n");
+    }
+  }
+  if (rs_open(device, &r.ctx)) die_lib("rs_open");
   uint32_t cus, lds;
   char name[256];
   rs_device_info(r.ctx, &cus, &lds, name, sizeof(name));
@@ -562,8 +604,9 @@ int sieve(Options o) {
     if (dt < 0.5 && c.nfactors < r.factor_cap / 8 && chunk < (1ull << 62)) chunk *= 2;
     else if (dt > 2.0 && chunk > (1u << 16)) chunk /= 2;
     const time_t now = time(nullptr);
-    if (now - last_prog > 1) {
+    if (now - last_prog > 1) {  // every 2 seconds (cl_sieve.cpp:1406-1414)
       print_progress(st, &smooth, start_time, run_start_p);
+      if (!o.frac_file.empty()) rs_boinc::write_fraction_done(o.frac_file, rs_boinc::fraction_done(st.pmin, st.pmax, st.p));
       last_prog = now;
     }
     if (now - last_ckpt > 60 || c.nfactors > r.factor_cap / 2) {
@@ -574,6 +617,8 @@ int sieve(Options o) {
   st.p = st.pmax;
   get_results(r, true);
   finalize(r);
+  if (!o.frac_file.empty() && !rs_boinc::write_fraction_done(o.frac_file, 1.0))
+    fprintf(stderr, "Cannot write %s\n", o.frac_file.c_str());
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   fprintf(stderr, "Sieve complete.\nfactors %" PRIu64 ", prime count %" PRIu64 ", duplicate factors %" PRIu64 "\n",
           st.factorcount, st.primecount, st.dupcount);

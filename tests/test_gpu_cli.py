@@ -204,3 +204,52 @@ def test_cli_resume_mid_range(tmp_path):
     checkpoint(short, len(before) - 1)
     r = _run(args, str(short))
     assert r.returncode != 0 and "Missing factors" in r.stdout
+
+
+def test_cli_boinc_shim(tmp_path):
+    """The BOINC shim (cl_sieve.cpp:40-62, 1406-1414; main.cpp:207-229):
+    without -d the device comes from the slot's init_data.xml, -F leaves the
+    fraction done 1 at the end, and -T writes the trickle-up message at a
+    checkpoint once a day has passed since the checkpoint's last_trickle (a
+    resumed checkpoint from long ago); the factor file is the plain run's.  A
+    malformed device tag stops the run before the GPU is opened."""
+    import ctypes as C
+    base, ks, cs = 2, [3, 5, 7, 11], [1, -1, 1, -1]
+    a = _make_abcd(tmp_path / "in.abcd", base, ks, cs, 20000, 60000, 0.6, 5)
+    p0, p1 = 10**8, 10**8 + 10**6
+    args = ["-p", str(p0), "-P", str(p1), "-i", "in.abcd", "-f", "out.txt"]
+    plain = tmp_path / "plain"
+    plain.mkdir()
+    (plain / "in.abcd").write_text((tmp_path / "in.abcd").read_text())
+    r = _run(args, str(plain))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "init_data.xml not found, using device 0." in r.stdout
+
+    d = tmp_path / "boinc"
+    d.mkdir()
+    (d / "in.abcd").write_text((tmp_path / "in.abcd").read_text())
+    (d / "init_data.xml").write_text("<app_init_data>\n<gpu_type>ATI</gpu_type>\n<gpu_device_num>0</gpu_device_num>\n"
+                                     "</app_init_data>\n")
+    st = R.State()
+    st.pmin, st.pmax, st.p = p0, p1, p0
+    st.base, st.nmin, st.nmax, st.kcount = base, a.nmin, a.nmax, len(ks)
+    for i in range(len(ks)):
+        st.klist[i] = int(a.k[i]) if int(a.c[i]) > 0 else -int(a.k[i])
+    st.last_trickle = 1  # the checkpoint's last trickle, long ago
+    assert R.lib().rs_state_write(str(d).encode(), st, C.c_int(1)) == 0
+    (d / "out.txt").write_text("")
+    r = _run(args + ["-F", "frac.txt", "-T", "trickle.xml"], str(d))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Using device 0 from init_data.xml" in r.stdout
+    assert (d / "frac.txt").read_text() == "1.000000\n"
+    msg = (d / "trickle.xml").read_text()
+    assert msg.startswith("<trickle_up>\n   <progress>") and msg.endswith("</runtime>\n</trickle_up>\n")
+    assert "<cputime>" in msg
+    assert sorted(_factor_lines(d / "out.txt")) == sorted(_factor_lines(plain / "out.txt"))
+
+    bad = tmp_path / "bad"
+    bad.mkdir()
+    (bad / "in.abcd").write_text((tmp_path / "in.abcd").read_text())
+    (bad / "init_data.xml").write_text("<gpu_device_num>gpu0</gpu_device_num>\n")
+    r = _run(args, str(bad))
+    assert r.returncode != 0 and "init_data.xml holds no valid gpu_device_num" in r.stdout
