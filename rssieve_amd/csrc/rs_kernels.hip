@@ -1,17 +1,20 @@
 // rs_kernels.hip -- HIP kernels for the k*b^n+-1 BSGS sieve on gfx950.
 //
 // Work is organised in rounds over a batch of jobs (one job is a batch of
-// one).  A round is five launches on one stream that need no host input:
+// one).  A round is six launches on one stream that need no host input:
 //
 //  round_begin  advance each job's device-side segment cursor, emit the
 //               primes <= 113 (addsmallprimes.cl:8-27), find the wheel offset
 //               (cl_sieve.cpp:453-512)
 //  primegen     all base-2 strong probable primes of each job's segment with
 //               no prime factor <= 113 (getsegprimes.cl:128-182)
-//  setup        one wavefront per prime: Montgomery constants, batch inversion
-//               of the k by wavefront product scans, the power-residue /
-//               quadratic-character prefilter (setup.cl:11-209) and per-class
-//               compaction (the reference's sort.cl, folded in)
+//  setup_pre    one lane per prime: Montgomery constants, the invertibility
+//               test and the power-residue tests of the base, the powers of
+//               the base the tables need (setup.cl:11-165)
+//  setup        one wavefront per prime: batch inversion of the k by
+//               wavefront product scans, the power-residue / quadratic-
+//               character prefilter of every k (setup.cl:166-209) and
+//               per-class compaction (the reference's sort.cl, folded in)
 //  round_items  prefix sums of the work items of every job
 //  bsgs         persistent workgroups pull (job, prime, class) items from one
 //               device queue; the baby steps fill a blocked Bloom filter in
@@ -21,6 +24,7 @@
 //
 // Because the cursors live on the device, the round can be captured once in
 // a HIP graph and replayed until every job's range is done.
+#include <cstddef>
 #include <mutex>
 #include <type_traits>
 
@@ -390,6 +394,8 @@ struct MontArith {
   }
   __device__ uint64_t mont(V x) const { return x; }
   __device__ uint64_t bits(V x) const { return x; }  // a canonical integer key
+  __device__ uint64_t store(V x) const { return x; }  // a value as staged between the setup kernels
+  __device__ V load(uint64_t u) const { return u; }
 };
 
 struct FpArith {
@@ -422,17 +428,229 @@ struct FpArith {
   }
   __device__ uint64_t mont(V x) const { return fp_to_u64(fp_mulmod(x, R, p, pinv)); }
   __device__ uint64_t bits(V x) const { return fp_to_u64(x); }
+  __device__ uint64_t store(V x) const { return (uint64_t)__double_as_longlong(x); }
+  __device__ V load(uint64_t u) const { return __longlong_as_double((long long)u); }
 };
 
-// The setup of prime p = primes[pi] by one wavefront (setup.cl:89-209 and
-// sort.cl:11-104); zv and zh are the wavefront's LDS for the z^i table.
+// ---------------------------------------------------------------------------
+// Setup, first kernel (k_setup_pre): the values that depend on the prime and
+// the job's constants but on no single k, one lane per prime.  The second
+// kernel (k_setup, one wavefront per prime) spends its lanes on the k; these
+// per-prime exponentiations in its lanes would keep most of them idle (an
+// exponent per lane for 18 of 64 lanes).  The values are staged in the
+// prime's own record slot, which k_setup overwrites with the final record.
+// ---------------------------------------------------------------------------
+struct PreRec {
+  uint64_t p, q, one;
+  uint64_t v[8];             // residues as the arithmetic stages them (store / load), slots kPv*
+  uint32_t nk[RS_NCLASS];    // 0, 0, 0: the final record's "no work" for a prime skipped here
+  uint32_t info;             // kPreSkip | passing power-residue tests (bit i: g_prd[i]) << 1 |
+                             // (quadratic character of b + 1) << 13
+  uint32_t La, Lb, Lc;       // gcd(p - 1, 80), gcd(p - 1, 63), gcd(p - 1, 143) (residue-class primes)
+  uint32_t pad32;
+  uint64_t pad;
+};
+static_assert(sizeof(PreRec) == sizeof(PrimeRec), "PreRec fills a PrimeRec slot");
+static_assert(offsetof(PreRec, nk) == offsetof(PrimeRec, nk), "a skipped prime's staged nk is its final nk");
+constexpr uint32_t kPreSkip = 1u;
+// slots of PreRec::v: (prod of the k)^-1, b^-1, b^NMIN; then for residue
+// classes za, zb, zc, otherwise b^-Q, b^-QT, b^-2QT, b^T, b^2T (a small
+// prime of a residue-class job: b^-2Q, b^-2QT for its parity split)
+enum { kPvInvK = 0, kPvBinv = 1, kPvBn = 2, kPvZa = 3, kPvZb = 4, kPvZc = 5,
+       kPvG0 = 3, kPvGT0 = 4, kPvGT1 = 5, kPvBT0 = 6, kPvBT1 = 7 };
+
+// gcd(p - 1, 80), gcd(p - 1, 63), gcd(p - 1, 143)
 template <class A>
-__device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64_t p, uint64_t q, uint64_t one,
-                                            uint32_t pi, uint32_t lane, uint64_t* zv, uint32_t* zh) {
+__device__ __forceinline__ void class_moduli(const A& ar, uint64_t pm, uint32_t* La, uint32_t* Lb, uint32_t* Lc) {
+  uint64_t qt;
+  const uint32_t v2 = (uint32_t)__builtin_ctzll(pm);
+  *La = (1u << (v2 < 4 ? v2 : 4)) * (ar.divides(pm, 5, 1.0 / 5, &qt) ? 5u : 1u);
+  *Lb = (ar.divides(pm, 9, 1.0 / 9, &qt) ? 9u : ar.divides(pm, 3, 1.0 / 3, &qt) ? 3u : 1u) *
+        (ar.divides(pm, 7, 1.0 / 7, &qt) ? 7u : 1u);
+  *Lc = (ar.divides(pm, 11, 1.0 / 11, &qt) ? 11u : 1u) * (ar.divides(pm, 13, 1.0 / 13, &qt) ? 13u : 1u);
+}
+
+template <class A>
+__device__ __forceinline__ void pre_prime(const A& ar, const SegJob& J, uint64_t p, uint64_t q, uint64_t one,
+                                          PreRec* __restrict__ out) {
+  using V = typename A::V;
+  const JobDev* __restrict__ job = J.job;
+  const uint32_t base = job->base;
+  const int K = job->kcount;
+  const uint32_t algo = job->algo;
+  const uint32_t T = J.threads;
+  const V vone = ar.one(), nmo = ar.neg_one();
+  const V mbase = ar.base(base);
+  PreRec o;
+  o.p = p;
+  o.q = q;
+  o.one = one;
+  o.nk[0] = o.nk[1] = o.nk[2] = 0;
+  o.La = 2;
+  o.Lb = o.Lc = 1;
+  o.pad32 = 0;
+  o.pad = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) o.v[i] = 0;
+  // The invertibility test (setup.cl:137-162): total = (prod of the k) *
+  // b^Qref, Qref being the reference's own split for this n-range (for a
+  // composite 2-PRP the outcome depends on that exponent, so every BSGS shape
+  // uses the same one), against its Fermat inverse total^(p-2).  The product
+  // of the k in two chains (a canonical residue, whatever the order).
+  V P0 = vone, P1 = vone;
+  int i = 0;
+  for (; i + 1 < K; i += 2) {
+    P0 = ar.mul(P0, ar.from_int(job->ak[i]));
+    P1 = ar.mul(P1, ar.from_int(job->ak[i + 1]));
+  }
+  if (i < K) P0 = ar.mul(P0, ar.from_int(job->ak[i]));
+  const V P = ar.mul(P0, P1);
+  const V bQ1 = ar.base_pow(mbase, job->Qref - 1, base);  // b^(Qref-1), for b^-1 below
+  const V gQ = ar.mul(bQ1, mbase);
+  const V total = ar.mul(P, gQ);
+  const V inv_total = ar.pow_lg(total, p - 2);  // setup.cl:154
+  if (ar.mul(total, inv_total) != vone) {       // setup.cl:156-162
+    o.info = kPreSkip;
+    *out = o;
+    atomicAdd((unsigned long long*)&J.cnt->skipped, 1ull);
+    return;
+  }
+  const V binv = ar.mul(bQ1, ar.mul(inv_total, P));  // b^(Qref-1) * b^-Qref (setup.cl:164-165)
+  o.v[kPvInvK] = ar.store(ar.mul(inv_total, gQ));
+  o.v[kPvBinv] = ar.store(binv);
+  o.v[kPvBn] = ar.store(ar.pow_u64(mbase, job->nmin));
+
+  // The power-residue tests b^((p-1)/d) = 1 for d | p - 1 (setup.cl:22-48)
+  // and the quadratic character b^((p-1)/2) (setup.cl:51-53).
+  const uint64_t pm = p - 1;
+  uint32_t dmask = 0;
+  V rg;
+  const bool split = algo == 2 && p > kClassMinP;
+  if (p > kClassMinP) {
+    // With L = gcd(p - 1, 720720) and z = b^((p-1)/L), b^((p-1)/d) = z^(L/d)
+    // for every d of the tests that divides 720720 (d | p - 1 <=> d | L);
+    // for 17, 19, 23 and 29, w = b^((p-1)/D) with D the product of those
+    // that divide p - 1 gives b^((p-1)/d) = w^(D/d).  The same residues as
+    // the direct exponents, in any ring: (p-1)/L and (p-1)/D are at least 4
+    // here, where pow_lg's masked shift is exact.
+    uint32_t La, Lb, Lc;
+    class_moduli(ar, pm, &La, &Lb, &Lc);
+    const uint32_t L = La * Lb * Lc;
+    uint64_t exL, qt;
+    ar.divides(pm, L, 1.0 / (double)L, &exL);
+    const V z = ar.pow_lg(mbase, exL);
+    uint32_t D = 1;
+#pragma unroll
+    for (int j = 8; j < 12; j++)
+      if (ar.divides(pm, g_prd[j], g_prdinv[j], &qt)) D *= g_prd[j];
+    V w = vone;
+    if (D > 1) {
+      uint64_t exD;
+      ar.divides(pm, D, 1.0 / (double)D, &exD);
+      w = ar.pow_lg(mbase, exD);
+    }
+#pragma unroll
+    for (int j = 0; j < 12; j++) {
+      const uint32_t d = g_prd[j];
+      if (j < 8 ? L % d == 0 : D % d == 0) {
+        const V r = j < 8 ? ar.pow_u64(z, L / d) : ar.pow_u64(w, D / d);
+        if (r == vone) dmask |= 1u << j;
+      }
+    }
+    rg = ar.pow_u64(z, L / 2);
+    if (split) {
+      // the bases of the residue-class tables: za = z^(Lb*Lc) = b^((p-1)/La),
+      // zb = z^(La*Lc), zc = z^(La*Lb) (1 when Lc = 1)
+      o.La = La;
+      o.Lb = Lb;
+      o.Lc = Lc;
+      o.v[kPvZa] = ar.store(ar.pow_u64(z, Lb * Lc));
+      o.v[kPvZb] = ar.store(ar.pow_u64(z, La * Lc));
+      o.v[kPvZc] = ar.store(Lc > 1 ? ar.pow_u64(z, La * Lb) : vone);
+    }
+  } else {
+    // small p: every exponent directly, as the reference takes it (an
+    // exponent of 1 gives pow_lg's masked-shift value, common.cl:52-53)
+#pragma unroll
+    for (int j = 0; j < 12; j++) {
+      uint64_t e;
+      if (ar.divides(pm, g_prd[j], g_prdinv[j], &e)) {
+        const V r = e ? ar.pow_lg(mbase, e) : vone;
+        if (r == vone) dmask |= 1u << j;
+      }
+    }
+    const uint64_t e2 = pm >> 1;
+    rg = e2 ? ar.pow_lg(mbase, e2) : vone;
+  }
+  const int resg = (rg == vone) - (rg == nmo);
+  o.info = dmask << 1 | (uint32_t)(resg + 1) << 13;
+  if (!split) {
+    // the constants of the reference's tables and of the merged table:
+    // b^T and b^2T (baby strides of a workgroup round), b^-Q (giant step)
+    // and its T-th and 2T-th powers
+    const uint32_t Q = job->Q;
+    o.v[kPvBT0] = ar.store(ar.pow_sm(mbase, T));
+    o.v[kPvBT1] = ar.store(ar.pow_sm(mbase, 2 * T));
+    if (algo == 2) {  // a small prime of a residue-class job: the parity split, table span 2Q
+      o.v[kPvG0] = ar.store(ar.pow_sm(binv, 2 * Q));
+      o.v[kPvGT0] = ar.store(ar.pow_sm(binv, 2 * Q * T));
+    } else {
+      o.v[kPvG0] = ar.store(ar.pow_sm(binv, Q));
+      o.v[kPvGT0] = ar.store(ar.pow_sm(binv, Q * T));
+      o.v[kPvGT1] = ar.store(ar.pow_sm(binv, 2 * Q * T));
+    }
+  }
+  *out = o;
+}
+
+// one lane per prime; every job of the batch on its own blockIdx.y
+__global__ __launch_bounds__(256) void k_setup_pre(SegJob* __restrict__ jobs) {
+  const SegJob& J = jobs[blockIdx.y];
+  Counters* __restrict__ cnt = J.cnt;
+  const uint32_t cap = J.cap;
+  const uint32_t ngen = J.nprimes;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicAdd((unsigned long long*)&cnt->primecount, (unsigned long long)ngen);
+    atomicMax((unsigned long long*)&cnt->max_seg_primes, (unsigned long long)ngen);
+    if (ngen > cap) atomicAdd((unsigned long long*)&cnt->seg_overflow, 1ull);
+  }
+  const uint32_t np = ngen < cap ? ngen : cap;
+  const uint32_t pi = blockIdx.x * 256 + threadIdx.x;
+  if (pi >= np) return;
+  const uint64_t p = J.primes[pi];
+  const uint64_t q = mont_inv(p);
+  const uint64_t one = r_mod(p);
+  PreRec* out = reinterpret_cast<PreRec*>(J.recs + pi);
+  if ((p & 1) && p < kFpLimit) {
+    FpArith ar;
+    ar.p = (double)p;
+    ar.pinv = 1.0 / ar.p;
+    ar.R = (double)one;
+    pre_prime(ar, J, p, q, one, out);
+  } else {
+    MontArith ar;
+    ar.p = p;
+    ar.q = q;
+    ar.one_ = one;
+    uint64_t r2 = madd(madd(one, one, p), madd(one, one, p), p);
+#pragma unroll
+    for (int i = 0; i < 5; i++) r2 = mmul(r2, r2, p, q);  // 4^(2^5) = 2^64: Montgomery form of R
+    ar.r2 = r2;
+    pre_prime(ar, J, p, q, one, out);
+  }
+}
+
+// The setup of prime pre.p = primes[pi] by one wavefront (setup.cl:89-209 and
+// sort.cl:11-104) from the values k_setup_pre staged; zv and zh are the
+// wavefront's LDS for the z^i table.
+template <class A>
+__device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const PreRec& pre, uint32_t pi,
+                                            uint32_t lane, uint64_t* zv, uint32_t* zh) {
   using V = typename A::V;
   const JobDev* __restrict__ job = J.job;
   PrimeRec* __restrict__ R = J.recs + pi;
   Counters* __restrict__ cnt = J.cnt;
+  const uint64_t p = pre.p, q = pre.q, one = pre.one;
   const uint32_t base = job->base;
   const int K = job->kcount;
   const uint32_t T = J.threads;
@@ -454,12 +672,6 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   // k for k = lane and k = lane + 64; 'one' pads the products.
   const V mk0 = (int)lane < K ? ar.from_int(job->ak[lane]) : vone;
   const V mk1 = (int)lane + 64 < K ? ar.from_int(job->ak[lane + 64]) : vone;
-  // The invertibility test multiplies the k by b^Qref, Qref being the
-  // reference's own split for this n-range (setup.cl:137-162): for a composite
-  // 2-PRP the test's outcome depends on that exponent, so every BSGS shape
-  // uses the same one and skips the same numbers.
-  const V bQ1 = ar.base_pow(mbase, job->Qref - 1, base);  // b^(Qref-1), for b^-1 below
-  const V gQ = ar.mul(bQ1, mbase);
 
   // inclusive prefix (s) and suffix (u) products over the wavefront
   V s0 = mk0, s1 = mk1, u0 = mk0, u1 = mk1;
@@ -484,8 +696,12 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   const V pre1 = ar.mul(A0, lane ? ep1 : vone);
   const V suf0 = ar.mul(lane < 63 ? es0 : vone, B0);
   const V suf1 = lane < 63 ? es1 : vone;
-  const V AB = ar.mul(A0, B0);
-  const V total = ar.mul(AB, gQ);  // prefix[KCOUNT] of setup.cl:147-151
+
+  // the values of k_setup_pre: the prime passed the invertibility test
+  const V inv_k = ar.load(pre.v[kPvInvK]);  // (prod of all k)^-1
+  const V binv = ar.load(pre.v[kPvBinv]);   // b^-1
+  const uint64_t dmask = (pre.info >> 1) & 0xFFFu;  // passing d tests, bit i = g_prd[i]
+  const int resg = (int)((pre.info >> 13) & 3u) - 1;
 
   // residue classes: L = La * Lb * Lc = gcd(p - 1, 80) * gcd(p - 1, 63) * gcd(p - 1, 143)
   const uint64_t pm = p - 1;
@@ -493,59 +709,12 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   uint32_t L = 2, La = 2, Lb = 1, Lc = 1;
   uint64_t exL = 0;
   if (split) {
-    const uint32_t v2 = (uint32_t)__builtin_ctzll(pm);
-    uint64_t qt;
-    La = (1u << (v2 < 4 ? v2 : 4)) * (ar.divides(pm, 5, 1.0 / 5, &qt) ? 5u : 1u);
-    Lb = (ar.divides(pm, 9, 1.0 / 9, &qt) ? 9u : ar.divides(pm, 3, 1.0 / 3, &qt) ? 3u : 1u) *
-         (ar.divides(pm, 7, 1.0 / 7, &qt) ? 7u : 1u);
-    Lc = (ar.divides(pm, 11, 1.0 / 11, &qt) ? 11u : 1u) * (ar.divides(pm, 13, 1.0 / 13, &qt) ? 13u : 1u);
+    La = pre.La;
+    Lb = pre.Lb;
+    Lc = pre.Lc;
     L = La * Lb * Lc;
     ar.divides(pm, L, 1.0 / (double)L, &exL);
   }
-
-  // Lane-parallel exponentiations, one code path (no serialized branches):
-  //  lane 0: Fermat inverse total^(p-2)          (setup.cl:154)
-  //  lanes 1..12: base^((p-1)/d) for d | p-1       (setup.cl:22-48)
-  //  lane 13: base^((p-1)/2)                       (setup.cl:51-53)
-  //  lanes 14, 15, 16: the residue classes' za = base^((p-1)/La),
-  //    zb = base^((p-1)/Lb) and zc = base^((p-1)/Lc) (with Lc > 1)
-  //  lane 17: base^NMIN (residue classes)
-  uint64_t ex = 0;
-  V x = mbase;
-  bool dvalid = false;
-  if (lane == 0) {
-    ex = p - 2;
-    x = total;
-  } else if (lane <= 12) {
-    dvalid = ar.divides(pm, g_prd[lane - 1], g_prdinv[lane - 1], &ex);
-    if (!dvalid) ex = 0;
-  } else if (lane == 13) {
-    ex = pm >> 1;
-  } else if (split && lane >= 14 && lane <= 16) {
-    // La, Lb, Lc divide p - 1 (one exact quotient for the three lanes)
-    const uint32_t Lz = lane == 14 ? La : lane == 15 ? Lb : Lc;
-    if (Lz > 1 || lane != 16) ar.divides(pm, Lz, 1.0 / (double)Lz, &ex);
-  } else if (split && lane == 17) {
-    ex = job->nmin;
-  }
-  V r = vone;
-  if (ex) r = ar.pow_lg(x, ex);
-  if (ex == 1 && lane == 17) r = mbase;  // the masked shift of pow_lg needs an exponent >= 2
-  const V inv_total = shfl64(r, 0);
-  const bool invertible = ar.mul(total, inv_total) == vone;  // setup.cl:156-162
-  const uint64_t dmask = __ballot(lane <= 12 && dvalid && r == vone) >> 1;  // passing d tests, bit i = g_prd[i]
-  const V rg = shfl64(r, 13);
-  const int resg = (rg == vone) - (rg == nmo);
-
-  if (!invertible) {
-    if (lane == 0) {
-      R->nk[0] = R->nk[1] = R->nk[2] = 0;
-      atomicAdd((unsigned long long*)&cnt->skipped, 1ull);
-    }
-    return;
-  }
-  const V gQref_inv = ar.mul(inv_total, AB);  // b^-Qref, setup.cl:164-165
-  const V inv_k = ar.mul(inv_total, gQ);      // (prod of all k)^-1
 
   // k^-1 and the target hadj for both slots (setup.cl:180-188)
   const V kinv0 = ar.mul(inv_k, ar.mul(pre0, suf0));
@@ -595,8 +764,13 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     dm &= dm - 1;
     int t0, t1;
     const uint32_t d = g_prd[i];
-    if (split && L % d == 0) sign_from_y(L / d, &t0, &t1);
-    else pow_sign2(shfl64(ex, i + 1), &t0, &t1);
+    if (split && L % d == 0) {
+      sign_from_y(L / d, &t0, &t1);
+    } else {
+      uint64_t e;
+      ar.divides(pm, d, g_prdinv[i], &e);
+      pow_sign2(e, &t0, &t1);
+    }
     if (t0 != 1) cls0 = 0;
     if (t1 != 1) cls1 = 0;
   }
@@ -627,22 +801,14 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
   }
   const bool merged = algo != 0;
 
-  // Constants of the BSGS kernel, lane-parallel:
-  //  lanes 0, 3, 4: b^NMIN, b^T, b^(2T); b^-1 = b^(Qref-1) * b^-Qref
-  // With the residue classes b^NMIN came with the first round and the
-  // classes' constants follow below, so this round is skipped.
-  V y = vone, bn = vone, bT0 = vone, bT1 = vone, binv;
-  uint32_t ye = 0;
+  // Constants of the BSGS kernel: b^NMIN, and the baby strides b^T, b^(2T)
+  // of the reference's tables and the merged table, from k_setup_pre
+  const V bn = ar.load(pre.v[kPvBn]);
+  V bT0 = vone, bT1 = vone;
   if (!split) {
-    if (lane == 0) ye = job->nmin;
-    else if (lane == 3) ye = T;
-    else if (lane == 4) ye = 2 * T;
-    if (lane == 0 || lane == 3 || lane == 4) y = ar.pow_sm(mbase, ye);
-    bn = shfl64(y, 0);
-    bT0 = shfl64(y, 3);
-    bT1 = shfl64(y, 4);
+    bT0 = ar.load(pre.v[kPvBT0]);
+    bT1 = ar.load(pre.v[kPvBT1]);
   }
-  binv = ar.mul(bQ1, gQref_inv);
 
   uint32_t nk[3] = {0, 0, 0};
   uint32_t cS = 0, cE = 0, cM = 0;  // residue-class shape of this prime
@@ -653,7 +819,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     // and zc^i (i <= Lc, at zv[192..]), and their orders sa, sb, sc (0: no
     // i > 0 with value 1 in range, possible only for a composite p)
     for (uint32_t h = lane; h < kZH; h += 64) zh[h] = 0u;
-    const V za = shfl64(r, 14), zb = shfl64(r, 15), zc = Lc > 1 ? shfl64(r, 16) : vone;
+    const V za = ar.load(pre.v[kPvZa]), zb = ar.load(pre.v[kPvZb]), zc = ar.load(pre.v[kPvZc]);
     // z^i for i <= Lz at zv[vb + i], 64 per pass; the order of z in range
     auto table = [&](V zz, uint32_t Lz, uint32_t vb) -> uint32_t {
       uint32_t sz = 0;
@@ -848,20 +1014,13 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
     gT0v = gT1v = shfl64(y2T, 0);
     bTa = shfl64(y2T, 1);
     bTb = shfl64(y2, 1);
-    bn = shfl64(r, 17);
   } else {
-    //  lanes 0, 1, 2: b^-Q, b^-QT, b^-2QT (giant step of the table span Q)
-    ye = 0;
-    if (lane == 0) ye = job->Q;
-    else if (lane == 1) ye = job->Q * T;
-    else if (lane == 2) ye = 2 * job->Q * T;
-    y = vone;
-    if (lane < 3) y = ar.pow_sm(binv, ye);
-    const V gQ_inv = shfl64(y, 0), gT0 = shfl64(y, 1), gT1 = shfl64(y, 2);
+    // b^-Q, b^-QT, b^-2QT (giant step of the table span Q) from k_setup_pre
+    const V gQ_inv = ar.load(pre.v[kPvG0]);
     g0v = gQ_inv;
-    g1v = ar.mul(gQ_inv, gQ_inv);
-    gT0v = gT0;
-    gT1v = gT1;
+    g1v = algo == 2 ? gQ_inv : ar.mul(gQ_inv, gQ_inv);
+    gT0v = ar.load(pre.v[kPvGT0]);
+    gT1v = algo == 2 ? gT0v : ar.load(pre.v[kPvGT1]);
     bTa = merged ? bT1 : bT0;
     bTb = bT1;
     if (!merged) {
@@ -920,14 +1079,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
         cS = 2;
         cE = job->Q;
         cM = (uint32_t)(((uint64_t)job->nmax - job->nmin + 1 + 2ull * cE - 1) / (2ull * cE));
-        const uint32_t sE = 2 * cE;
-        ye = 0;
-        if (lane == 0) ye = sE;
-        else if (lane == 1) ye = sE * T;
-        y = vone;
-        if (lane < 2) y = ar.pow_sm(binv, ye);
-        g0v = g1v = shfl64(y, 0);
-        gT0v = gT1v = shfl64(y, 1);
+        // (g0v and gT0v are b^-2Q and b^-2QT here, staged for this case)
         bTb = ar.mul(mbase, mbase);  // the table stride b^S, S = 2
       }
     }
@@ -974,23 +1126,27 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, uint64
 // needs the waves to cover its FP64 and LDS latencies
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_setup(SegJob* __restrict__ jobs) {
   const SegJob& J = jobs[blockIdx.y];
-  Counters* __restrict__ cnt = J.cnt;
-  const uint32_t cap = J.cap;
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t pi = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const uint32_t ngen = J.nprimes;
+  const uint32_t pi = rfl(blockIdx.x * 4 + (threadIdx.x >> 6));
   __shared__ uint64_t s_zv[4][kZV];  // per wavefront: z^i of the residue classes
   __shared__ uint32_t s_zh[4][kZH];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    atomicAdd((unsigned long long*)&cnt->primecount, (unsigned long long)ngen);
-    atomicMax((unsigned long long*)&cnt->max_seg_primes, (unsigned long long)ngen);
-    if (ngen > cap) atomicAdd((unsigned long long*)&cnt->seg_overflow, 1ull);
-  }
+  const uint32_t ngen = J.nprimes, cap = J.cap;
   const uint32_t np = ngen < cap ? ngen : cap;
   if (pi >= np) return;
-  const uint64_t p = J.primes[pi];
-  const uint64_t q = mont_inv(p);
-  const uint64_t one = r_mod(p);
+  // the staged values, by scalar loads (one address per wavefront)
+  const PreRec* __restrict__ src = reinterpret_cast<const PreRec*>(J.recs + pi);
+  PreRec pre;
+  pre.info = ld_const(&src->info);
+  if (pre.info & kPreSkip) return;  // not invertible: k_setup_pre counted it and wrote nk = 0
+  pre.p = ld_const(&src->p);
+  pre.q = ld_const(&src->q);
+  pre.one = ld_const(&src->one);
+#pragma unroll
+  for (int i = 0; i < 8; i++) pre.v[i] = ld_const(&src->v[i]);
+  pre.La = ld_const(&src->La);
+  pre.Lb = ld_const(&src->Lb);
+  pre.Lc = ld_const(&src->Lc);
+  const uint64_t p = pre.p, q = pre.q, one = pre.one;
   uint64_t* zv = s_zv[threadIdx.x >> 6];
   uint32_t* zh = s_zh[threadIdx.x >> 6];
   if ((p & 1) && p < kFpLimit) {
@@ -998,7 +1154,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     ar.p = (double)p;
     ar.pinv = 1.0 / ar.p;
     ar.R = (double)one;
-    setup_prime(ar, J, p, q, one, pi, lane, zv, zh);
+    setup_prime(ar, J, pre, pi, lane, zv, zh);
   } else {
     MontArith ar;
     ar.p = p;
@@ -1008,7 +1164,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 #pragma unroll
     for (int i = 0; i < 5; i++) r2 = mmul(r2, r2, p, q);  // 4^(2^5) = 2^64: Montgomery form of R
     ar.r2 = r2;
-    setup_prime(ar, J, p, q, one, pi, lane, zv, zh);
+    setup_prime(ar, J, pre, pi, lane, zv, zh);
   }
 }
 
@@ -1998,6 +2154,8 @@ hipError_t launch_round_prep(SegJob* jobs, BatchState* bs, uint32_t njobs, uint3
                              hipStream_t s) {
   hipLaunchKernelGGL(k_round_begin, dim3((njobs + 63) / 64), dim3(64), 0, s, jobs, bs, njobs);
   if (gen_blocks) hipLaunchKernelGGL(k_primegen, dim3(gen_blocks, njobs), dim3(256), 0, s, jobs);
+  // k_setup takes four primes per workgroup, k_setup_pre 256
+  hipLaunchKernelGGL(k_setup_pre, dim3((4 * setup_blocks + 255) / 256, njobs), dim3(256), 0, s, jobs);
   hipLaunchKernelGGL(k_setup, dim3(setup_blocks, njobs), dim3(256), 0, s, jobs);
   hipLaunchKernelGGL(k_round_items, dim3(1), dim3(64), 0, s, jobs, bs, njobs);
   return hipGetLastError();
