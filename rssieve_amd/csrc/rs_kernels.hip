@@ -315,8 +315,17 @@ __constant__ uint32_t g_prd[12] = {3, 4, 5, 7, 8, 9, 11, 13, 17, 19, 23, 29};
 // searched, and the classes only narrow the n they are searched on, so the
 // solutions found are the same.  For a prime p with b a quadratic
 // non-residue s is even and fixes the parity the reference's class gives.
-constexpr uint32_t kZV = 336;                 // za^i at [0, 81), zb^i at [128, 192), zc^i at [192, 336)
-constexpr uint32_t kZH = 640;                 // their hash slots: [0, 256), [256, 384) and [384, 640)
+// A fourth part Ld, one prime r in [17, 127] that divides p - 1, with zd =
+// b^((p-1)/Ld) != 1 (so of order r for a prime p) and yd = h^((p-1)/Ld),
+// narrows n further mod r (the largest such r whose stride stays below 2^24,
+// the shifts' width in the BSGS queue; 1 when none): L = La*Lb*Lc*Ld.
+constexpr uint32_t kRd[25] = {17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67,
+                              71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113, 127};
+__constant__ uint32_t g_rd[25] = {17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67,
+                                  71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113, 127};
+constexpr uint32_t kZV = 464;                 // za^i at [0, 81), zb^i at [128, 192), zc^i at [192, 336),
+                                              // zd^i at [336, 464)
+constexpr uint32_t kZH = 896;                 // their hash slots: [0, 256), [256, 384), [384, 640), [640, 896)
 constexpr uint64_t kClassMinP = 1u << 20;     // smaller p keep the merged table's parity split; above it
                                               // the exponent (p-1)/L is at least 2 (L <= 720720)
 
@@ -446,17 +455,17 @@ struct PreRec {
   uint32_t nk[RS_NCLASS];    // 0, 0, 0: the final record's "no work" for a prime skipped here
   uint32_t info;             // kPreSkip | passing power-residue tests (bit i: g_prd[i]) << 1 |
                              // (quadratic character of b + 1) << 13
-  uint32_t La, Lb, Lc;       // gcd(p - 1, 80), gcd(p - 1, 63), gcd(p - 1, 143) (residue-class primes)
-  uint32_t pad32;
+  uint32_t La, Lb, Lc, Ld;   // gcd(p - 1, 80), gcd(p - 1, 63), gcd(p - 1, 143) and the fourth part
+                             // (residue-class primes)
   uint64_t pad;
 };
 static_assert(sizeof(PreRec) == sizeof(PrimeRec), "PreRec fills a PrimeRec slot");
 static_assert(offsetof(PreRec, nk) == offsetof(PrimeRec, nk), "a skipped prime's staged nk is its final nk");
 constexpr uint32_t kPreSkip = 1u;
 // slots of PreRec::v: (prod of the k)^-1, b^-1, b^NMIN; then for residue
-// classes za, zb, zc, otherwise b^-Q, b^-QT, b^-2QT, b^T, b^2T (a small
+// classes za, zb, zc, zd, otherwise b^-Q, b^-QT, b^-2QT, b^T, b^2T (a small
 // prime of a residue-class job: b^-2Q, b^-2QT for its parity split)
-enum { kPvInvK = 0, kPvBinv = 1, kPvBn = 2, kPvZa = 3, kPvZb = 4, kPvZc = 5,
+enum { kPvInvK = 0, kPvBinv = 1, kPvBn = 2, kPvZa = 3, kPvZb = 4, kPvZc = 5, kPvZd = 6,
        kPvG0 = 3, kPvGT0 = 4, kPvGT1 = 5, kPvBT0 = 6, kPvBT1 = 7 };
 
 // gcd(p - 1, 80), gcd(p - 1, 63), gcd(p - 1, 143)
@@ -487,8 +496,7 @@ __device__ __forceinline__ void pre_prime(const A& ar, const SegJob& J, uint64_t
   o.one = one;
   o.nk[0] = o.nk[1] = o.nk[2] = 0;
   o.La = 2;
-  o.Lb = o.Lc = 1;
-  o.pad32 = 0;
+  o.Lb = o.Lc = o.Ld = 1;
   o.pad = 0;
 #pragma unroll
   for (int i = 0; i < 8; i++) o.v[i] = 0;
@@ -536,37 +544,58 @@ __device__ __forceinline__ void pre_prime(const A& ar, const SegJob& J, uint64_t
     uint32_t La, Lb, Lc;
     class_moduli(ar, pm, &La, &Lb, &Lc);
     const uint32_t L = La * Lb * Lc;
-    uint64_t exL, qt;
+    uint64_t exL;
     ar.divides(pm, L, 1.0 / (double)L, &exL);
     const V z = ar.pow_lg(mbase, exL);
+    // D: the primes r of g_rd that divide p - 1 (17, 19, 23 and 29 first,
+    // the reference's tests; the others while D stays below 2^32)
     uint32_t D = 1;
+    uint32_t dm = 0;  // bit j: kRd[j] divides D
 #pragma unroll
-    for (int j = 8; j < 12; j++)
-      if (ar.divides(pm, g_prd[j], g_prdinv[j], &qt)) D *= g_prd[j];
+    for (int j = 0; j < 25; j++) {
+      uint64_t qt;
+      if (ar.divides(pm, kRd[j], 1.0 / kRd[j], &qt) && (j < 4 || (uint64_t)D * kRd[j] < (1ull << 32))) {
+        D *= kRd[j];
+        dm |= 1u << j;
+      }
+    }
     V w = vone;
     if (D > 1) {
       uint64_t exD;
       ar.divides(pm, D, 1.0 / (double)D, &exD);
       w = ar.pow_lg(mbase, exD);
     }
+    constexpr uint32_t kPrd8[8] = {3, 4, 5, 7, 8, 9, 11, 13};  // the tests' d that divide 720720
 #pragma unroll
-    for (int j = 0; j < 12; j++) {
-      const uint32_t d = g_prd[j];
-      if (j < 8 ? L % d == 0 : D % d == 0) {
-        const V r = j < 8 ? ar.pow_u64(z, L / d) : ar.pow_u64(w, D / d);
-        if (r == vone) dmask |= 1u << j;
+    for (int j = 0; j < 8; j++)
+      if (L % kPrd8[j] == 0 && ar.pow_u64(z, L / kPrd8[j]) == vone) dmask |= 1u << j;
+    // b^((p-1)/r) = w^(D/r) for r | D: the tests of 17..29, and the fourth
+    // class part (the largest r with b^((p-1)/r) != 1 and 2*L*r < 2^24)
+    uint32_t Ld = 1;
+    V zd = vone;
+    for (uint32_t m = dm; m; m &= m - 1) {
+      const int j = __ffs((int)m) - 1;
+      const uint32_t r = g_rd[j];
+      const V t = ar.pow_u64(w, D / r);
+      if (t == vone) {
+        if (j < 4) dmask |= 1u << (8 + j);
+      } else if (2ull * L * r < (1ull << 24)) {
+        Ld = r;
+        zd = t;
       }
     }
     rg = ar.pow_u64(z, L / 2);
     if (split) {
       // the bases of the residue-class tables: za = z^(Lb*Lc) = b^((p-1)/La),
-      // zb = z^(La*Lc), zc = z^(La*Lb) (1 when Lc = 1)
+      // zb = z^(La*Lc), zc = z^(La*Lb) (1 when Lc = 1), zd = b^((p-1)/Ld)
       o.La = La;
       o.Lb = Lb;
       o.Lc = Lc;
+      o.Ld = Ld;
       o.v[kPvZa] = ar.store(ar.pow_u64(z, Lb * Lc));
       o.v[kPvZb] = ar.store(ar.pow_u64(z, La * Lc));
       o.v[kPvZc] = ar.store(Lc > 1 ? ar.pow_u64(z, La * Lb) : vone);
+      o.v[kPvZd] = ar.store(zd);
     }
   } else {
     // small p: every exponent directly, as the reference takes it (an
@@ -706,13 +735,14 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
   // residue classes: L = La * Lb * Lc = gcd(p - 1, 80) * gcd(p - 1, 63) * gcd(p - 1, 143)
   const uint64_t pm = p - 1;
   const bool split = algo == 2 && p > kClassMinP;
-  uint32_t L = 2, La = 2, Lb = 1, Lc = 1;
+  uint32_t L = 2, La = 2, Lb = 1, Lc = 1, Ld = 1;
   uint64_t exL = 0;
   if (split) {
     La = pre.La;
     Lb = pre.Lb;
     Lc = pre.Lc;
-    L = La * Lb * Lc;
+    Ld = pre.Ld;
+    L = La * Lb * Lc * Ld;
     ar.divides(pm, L, 1.0 / (double)L, &exL);
   }
 
@@ -731,20 +761,28 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     *t0 = (a0 == vone) - (a0 == nmo);
     *t1 = (a1 == vone) - (a1 == nmo);
   };
-  // With the residue classes, y = h^((p-1)/L) is needed anyway, and for d | L
-  // the character h^((p-1)/d) is y^(L/d) (the same value from a short
-  // exponent): the reference's quadratic test and its power-residue tests
-  // with d | L take it from y.
-  V y0 = vone, y1 = vone, ya0 = vone, ya1 = vone, yb0 = vone, yb1 = vone, yc0 = vone, yc1 = vone;
+  // With the residue classes, y = h^((p-1)/L) is needed anyway, and its
+  // parts ya = y^(L/La), yb = y^(L/Lb), yc = y^(L/Lc), yd = y^(L/Ld) give the
+  // characters h^((p-1)/d) for d | L as short powers (the same values): the
+  // reference's quadratic test and its power-residue tests with d | L take
+  // them from there.
+  V ya0 = vone, ya1 = vone, yb0 = vone, yb1 = vone, yc0 = vone, yc1 = vone, yd0 = vone, yd1 = vone;
   if (split) {
-    ar.pow_lg2(h0, h1, exL, &y0, &y1);
-    // u = y^Lc, ya = u^Lb, yb = u^La, yc = y^(La*Lb); Lc = 1 for most p
-    V u0 = y0, u1 = y1;
+    V t0, t1;
+    ar.pow_lg2(h0, h1, exL, &t0, &t1);  // y
+    if (Ld > 1) {
+      yd0 = ar.pow_u64(t0, La * Lb * Lc);
+      yd1 = ar.pow_u64(t1, La * Lb * Lc);
+      t0 = ar.pow_u64(t0, Ld);
+      t1 = ar.pow_u64(t1, Ld);
+    }
+    // on t = y^Ld: u = t^Lc, ya = u^Lb, yb = u^La, yc = t^(La*Lb); Lc = 1 for most p
+    V u0 = t0, u1 = t1;
     if (Lc > 1) {
-      u0 = ar.pow_u64(y0, Lc);
-      u1 = ar.pow_u64(y1, Lc);
-      yc0 = ar.pow_u64(y0, La * Lb);
-      yc1 = ar.pow_u64(y1, La * Lb);
+      u0 = ar.pow_u64(t0, Lc);
+      u1 = ar.pow_u64(t1, Lc);
+      yc0 = ar.pow_u64(t0, La * Lb);
+      yc1 = ar.pow_u64(t1, La * Lb);
     }
     ya0 = ar.pow_u64(u0, Lb);
     ya1 = ar.pow_u64(u1, Lb);
@@ -756,7 +794,6 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     *t0 = (a0 == vone) - (a0 == nmo);
     *t1 = (a1 == vone) - (a1 == nmo);
   };
-  auto sign_from_y = [&](uint32_t e, int* t0, int* t1) { sign_of(y0, y1, e, t0, t1); };
   int cls0 = act0 ? 1 : 0, cls1 = act1 ? 1 : 0;
   uint64_t dm = dmask;
   while (dm) {
@@ -764,8 +801,15 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     dm &= dm - 1;
     int t0, t1;
     const uint32_t d = g_prd[i];
-    if (split && L % d == 0) {
-      sign_from_y(L / d, &t0, &t1);
+    // h^((p-1)/d) = ya^(La/d), yb^(Lb/d), yc^(Lc/d) or yd for d of one part
+    if (split && La % d == 0) {
+      sign_of(ya0, ya1, La / d, &t0, &t1);
+    } else if (split && Lb % d == 0) {
+      sign_of(yb0, yb1, Lb / d, &t0, &t1);
+    } else if (split && Lc % d == 0) {
+      sign_of(yc0, yc1, Lc / d, &t0, &t1);
+    } else if (split && d == Ld) {
+      sign_of(yd0, yd1, 1, &t0, &t1);
     } else {
       uint64_t e;
       ar.divides(pm, d, g_prdinv[i], &e);
@@ -820,6 +864,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     // i > 0 with value 1 in range, possible only for a composite p)
     for (uint32_t h = lane; h < kZH; h += 64) zh[h] = 0u;
     const V za = ar.load(pre.v[kPvZa]), zb = ar.load(pre.v[kPvZb]), zc = ar.load(pre.v[kPvZc]);
+    const V zd = ar.load(pre.v[kPvZd]);
     // z^i for i <= Lz at zv[vb + i], 64 per pass; the order of z in range
     auto table = [&](V zz, uint32_t Lz, uint32_t vb) -> uint32_t {
       uint32_t sz = 0;
@@ -835,11 +880,13 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
       }
       return sz;
     };
-    // (Lc = 1 for most p: then sc = 1 and the third table is not used)
+    // (Lc = 1 for most p: then sc = 1 and the third table is not used; Ld = 1
+    // likewise for the fourth)
     const uint32_t sa = table(za, La, 0u), sb = table(zb, Lb, 128u), sc = Lc > 1 ? table(zc, Lc, 192u) : 1u;
+    const uint32_t sd = Ld > 1 ? table(zd, Ld, 336u) : 1u;
     // The CRT needs pairwise coprime orders.  For a prime p they divide 80,
-    // 63 and 143 and are coprime; a composite p may give other orders, which
-    // are then tested by Euclid.
+    // 63, 143 and the prime Ld and are coprime; a composite p may give other
+    // orders, which are then tested by Euclid.
     auto gcd = [](uint32_t g, uint32_t h2) {
       while (h2) {
         const uint32_t t = g % h2;
@@ -848,10 +895,11 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
       }
       return g;
     };
-    const uint32_t sab = sa * sb;
-    const bool coprime = sa && sb && sc &&
-                         ((80u % sa == 0 && 63u % sb == 0 && 143u % sc == 0) || (gcd(sa, sb) == 1 && gcd(sab, sc) == 1));
-    const uint32_t s = coprime ? sab * sc : 0u;
+    const uint32_t sab = sa * sb, sabc = sab * sc;
+    const bool coprime = sa && sb && sc && sd &&
+                         ((80u % sa == 0 && 63u % sb == 0 && 143u % sc == 0 && (sd == 1 || sd == Ld)) ||
+                          (gcd(sa, sb) == 1 && gcd(sab, sc) == 1 && gcd(sabc, sd) == 1));
+    const uint32_t s = coprime ? sabc * sd : 0u;
     wave_sync();
     if (s) {
       for (uint32_t idx = lane; idx < sa; idx += 64) {
@@ -866,11 +914,16 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
         uint32_t h = (uint32_t)zv[192 + idx] & 255u;
         while (atomicCAS(&zh[384 + h], 0u, idx + 1) != 0u) h = (h + 1) & 255u;
       }
+      for (uint32_t idx = lane; Ld > 1 && idx < sd; idx += 64) {
+        uint32_t h = (uint32_t)zv[336 + idx] & 255u;
+        while (atomicCAS(&zh[640 + h], 0u, idx + 1) != 0u) h = (h + 1) & 255u;
+      }
     }
     wave_sync();
-    // sa^-1 mod sb and (sa*sb)^-1 mod sc for the CRT
-    const SmallMod Mb(s ? sb : 1u), Mc(s ? sc : 1u);
+    // sa^-1 mod sb, (sa*sb)^-1 mod sc and (sa*sb*sc)^-1 mod sd for the CRT
+    const SmallMod Mb(s ? sb : 1u), Mc(s ? sc : 1u), Md(s ? sd : 1u);
     const uint32_t sainv = s ? inv_mod_lanes(sa, Mb, lane) : 0u, sabinv = s ? inv_mod_lanes(sab, Mc, lane) : 0u;
+    const uint32_t sabcinv = s && sd > 1 ? inv_mod_lanes(sabc, Md, lane) : 0u;
     // index of a key in one table (hash base hb, mask hm, values at vb), or -1
     auto lookup = [&](uint64_t key, uint32_t hb, uint32_t hm, uint32_t vb) -> int {
       for (uint32_t h = (uint32_t)key & hm;; h = (h + 1) & hm) {
@@ -879,29 +932,35 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
         if (zv[vb + sl - 1] == key) return (int)sl - 1;
       }
     };
-    // the x < s with z^x = y, from ya = y^(Lb*Lc), yb = y^(La*Lc) and
-    // yc = y^(La*Lb), or -1
-    auto dlog = [&](V yav, V ybv, V ycv) -> int {
+    // the x < s with z^x = y, from ya = y^(L/La), yb = y^(L/Lb), yc =
+    // y^(L/Lc) and yd = y^(L/Ld), or -1
+    auto dlog = [&](V yav, V ybv, V ycv, V ydv) -> int {
       const int xa = lookup(ar.bits(yav), 0u, 255u, 0u);
       if (xa < 0) return -1;
       const int xb = lookup(ar.bits(ybv), 256u, 127u, 128u);
       if (xb < 0) return -1;
-      int xc = 0;
+      int xc = 0, xd = 0;
       if (Lc > 1) {
         xc = lookup(ar.bits(ycv), 384u, 255u, 192u);
         if (xc < 0) return -1;
       }
+      if (Ld > 1) {
+        xd = lookup(ar.bits(ydv), 640u, 255u, 336u);
+        if (xd < 0) return -1;
+      }
       const uint32_t dx = Mb((uint32_t)xb + sb - Mb((uint32_t)xa));
       const uint32_t xab = (uint32_t)xa + sa * Mb(dx * sainv);
       const uint32_t dy = Mc((uint32_t)xc + sc - Mc(xab));
-      return (int)(xab + sab * Mc(dy * sabinv));
+      const uint32_t xabc = xab + sab * Mc(dy * sabinv);
+      const uint32_t dz = Md((uint32_t)xd + sd - Md(xabc));
+      return (int)(xabc + sabc * Md(dz * sabcinv));
     };
     // Table stride S: s when s is even (it fixes the parity) or no k has a
     // parity class here, else 2s (CRT of the class mod s and the parity);
     // 2 without a usable s.  Each target gets one or two residues n mod S.
     const bool anyPar = __ballot(cls0 >= 2 || cls1 >= 2) != 0;
     const uint32_t S = s == 0 ? 2u : ((s & 1) && anyPar) ? 2 * s : s;
-    auto classes = [&](int cls, V yav, V ybv, V ycv, uint32_t (&rr)[2]) -> int {
+    auto classes = [&](int cls, V yav, V ybv, V ycv, V ydv, uint32_t (&rr)[2]) -> int {
       if (!cls) return 0;
       if (s == 0) {
         if (cls == 1) {
@@ -912,7 +971,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
         rr[0] = (uint32_t)(cls - 2);
         return 1;
       }
-      const int xl = dlog(yav, ybv, ycv);
+      const int xl = dlog(yav, ybv, ycv, ydv);
       if (xl < 0) return 0;
       const uint32_t ux = (uint32_t)xl;
       if (S == s) {
@@ -929,10 +988,10 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
       return 1;
     };
     uint32_t ra[2] = {0, 0}, rb[2] = {0, 0};
-    const int na = classes(cls0, ya0, yb0, yc0, ra), nb = classes(cls1, ya1, yb1, yc1, rb);
+    const int na = classes(cls0, ya0, yb0, yc0, yd0, ra), nb = classes(cls1, ya1, yb1, yc1, yd1, rb);
     const SmallMod MS(S);
     const uint32_t nmod = job->nmin % S;
-    // b^-d for the shifts d < S <= 1441440 < 2^21, in base-64 digits: lane j
+    // b^-d for the shifts d < S < 2^24, in base-64 digits: lane j
     // holds b^-j, b^-(64j), b^-(4096j) and b^-(262144j) (j < 64; the last two
     // levels only when S needs them), so b^-d is one shuffle per digit and a
     // product between (taken with every lane active, outside the divergent
@@ -1146,6 +1205,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   pre.La = ld_const(&src->La);
   pre.Lb = ld_const(&src->Lb);
   pre.Lc = ld_const(&src->Lc);
+  pre.Ld = ld_const(&src->Ld);
   const uint64_t p = pre.p, q = pre.q, one = pre.one;
   uint64_t* zv = s_zv[threadIdx.x >> 6];
   uint32_t* zh = s_zh[threadIdx.x >> 6];

@@ -1,10 +1,12 @@
 """The residue-class rule of the setup's class tables (RS_ALGO_CLASSES),
 checked by brute force.
 
-For L = gcd(p - 1, 720720) and z = b^((p-1)/L), every n with b^n = h (mod p)
-satisfies z^n = y with y = h^((p-1)/L), so n = x (mod s) for the x < s = ord(z)
-with z^x = y, and there is no n when y is not a power of z.  The setup keeps
-the reference's own tests (setup.cl:59-87: the parity classes and the
+For L = gcd(p - 1, 720720) * Ld and z = b^((p-1)/L), every n with b^n = h
+(mod p) satisfies z^n = y with y = h^((p-1)/L), so n = x (mod s) for the
+x < s = ord(z) with z^x = y, and there is no n when y is not a power of z.
+Ld is one prime r in [17, 127] dividing p - 1 with b^((p-1)/r) != 1 (the
+largest whose stride 2 * gcd(p - 1, 720720) * r stays below 2^24), or 1.
+The setup keeps the reference's own tests (setup.cl:59-87: the parity classes and the
 power-residue tests) and only narrows the n searched to that class, so the
 solutions found in [nmin, nmax] must be exactly those of the reference's
 class.  Composite base-2 strong pseudoprimes are included: there the
@@ -18,6 +20,7 @@ import random
 import pytest
 
 PRD = [3, 4, 5, 7, 8, 9, 11, 13, 17, 19, 23, 29]  # setup.cl:22-48
+RD = [17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113, 127]
 
 
 def _sprp2(n):
@@ -67,32 +70,47 @@ def _class_residues(p, b, hs, clss):
     La = (1 << min(v2, 4)) * (5 if pm % 5 == 0 else 1)                       # gcd(p-1, 80)
     Lb = (9 if pm % 9 == 0 else 3 if pm % 3 == 0 else 1) * (7 if pm % 7 == 0 else 1)  # gcd(p-1, 63)
     Lc = (11 if pm % 11 == 0 else 1) * (13 if pm % 13 == 0 else 1)                    # gcd(p-1, 143)
-    L = La * Lb * Lc                                                           # gcd(p-1, 720720)
+    L0 = La * Lb * Lc                                                          # gcd(p-1, 720720)
+    # the fourth part: the primes of RD dividing p - 1 (17..29 always, the
+    # others while their product stays below 2^32), the largest r with
+    # b^((p-1)/r) != 1 and 2 * L0 * r < 2^24
+    D, Ld = 1, 1
+    for j, r in enumerate(RD):
+        if pm % r == 0 and (j < 4 or D * r < 2**32):
+            D *= r
+    for r in RD:
+        if D % r == 0 and pow(b, pm // r, p) != 1 and 2 * L0 * r < 2**24:
+            Ld = r
+    L = L0 * Ld
     z = pow(b, pm // L, p)
-    # n mod the orders of z^(Lb*Lc), z^(La*Lc) and z^(La*Lb) (each exact: the
-    # first i with z_a^i = 1), combined by CRT when they are pairwise coprime
-    # (always for a prime p); otherwise (a composite p) only the parity split
-    za, zb, zc = pow(z, Lb * Lc, p), pow(z, La * Lc, p), pow(z, La * Lb, p)
+    # n mod the orders of z^(L/La), z^(L/Lb), z^(L/Lc) and z^(L/Ld) (each
+    # exact: the first i with z_a^i = 1), combined by CRT when they are
+    # pairwise coprime (always for a prime p); otherwise (a composite p) only
+    # the parity split
+    za, zb, zc, zd = (pow(z, L // La, p), pow(z, L // Lb, p), pow(z, L // Lc, p), pow(z, L // Ld, p))
     sa = next((i for i in range(1, La + 1) if pow(za, i, p) == 1), 0)
     sb = next((i for i in range(1, Lb + 1) if pow(zb, i, p) == 1), 0)
-    # Lc = 1: no constraint mod sc = 1 and no third table (as the kernel)
+    # Lc = 1: no constraint mod sc = 1 and no third table (as the kernel); Ld likewise
     sc = 1 if Lc == 1 else next((i for i in range(1, Lc + 1) if pow(zc, i, p) == 1), 0)
-    coprime = math.gcd(sa, sb) == 1 and math.gcd(sa * sb, sc) == 1
-    s = sa * sb * sc if sa and sb and sc and coprime else 0
+    sd = 1 if Ld == 1 else next((i for i in range(1, Ld + 1) if pow(zd, i, p) == 1), 0)
+    coprime = math.gcd(sa, sb) == 1 and math.gcd(sa * sb, sc) == 1 and math.gcd(sa * sb * sc, sd) == 1
+    s = sa * sb * sc * sd if sa and sb and sc and sd and coprime else 0
     any_par = any(c >= 2 for c in clss)
     S = 2 if s == 0 else (2 * s if (s & 1) and any_par else s)
     ta = {pow(za, i, p): i for i in range(sa)} if s else {}
     tb = {pow(zb, i, p): i for i in range(sb)} if s else {}
     tc = {pow(zc, i, p): i for i in range(sc)} if s and Lc > 1 else {}
+    td = {pow(zd, i, p): i for i in range(sd)} if s and Ld > 1 else {}
 
     def dlog(h):
-        """x < s with z^x = y (its residues mod sa, sb and sc), or None."""
+        """x < s with z^x = y (its residues mod sa, sb, sc and sd), or None."""
         y = pow(h, pm // L, p)
-        xa, xb = ta.get(pow(y, Lb * Lc, p)), tb.get(pow(y, La * Lc, p))
-        xc = tc.get(pow(y, La * Lb, p)) if Lc > 1 else 0
-        if xa is None or xb is None or xc is None:
+        xa, xb = ta.get(pow(y, L // La, p)), tb.get(pow(y, L // Lb, p))
+        xc = tc.get(pow(y, L // Lc, p)) if Lc > 1 else 0
+        xd = td.get(pow(y, L // Ld, p)) if Ld > 1 else 0
+        if xa is None or xb is None or xc is None or xd is None:
             return None
-        return next(x for x in range(s) if x % sa == xa and x % sb == xb and x % sc == xc)
+        return next(x for x in range(s) if x % sa == xa and x % sb == xb and x % sc == xc and x % sd == xd)
     res = []
     for h, cls in zip(hs, clss):
         rr = []
@@ -129,7 +147,7 @@ def test_pseudoprimes_are_present():
 def test_residue_classes_keep_the_reference_solutions(base):
     comp, prime = _cases(60000, 40, base)
     rng = random.Random(base)
-    narrowed = checked = 0
+    narrowed = checked = fourth = 0
     for p in comp + prime:
         if math.gcd(base, p) != 1:
             continue
@@ -151,4 +169,5 @@ def test_residue_classes_keep_the_reference_solutions(base):
             assert ours == ref, (p, base, h, cls, S, rr)
             checked += 1
             narrowed += len(rr) * 2 < S * (2 if cls == 1 else 1)
-    assert checked > 100 and narrowed > checked // 4
+            fourth += any(S % r == 0 for r in RD)
+    assert checked > 100 and narrowed > checked // 4 and fourth > checked // 10
