@@ -48,6 +48,7 @@ __device__ __forceinline__ T ld_global(const T* p) {
   return *(const __attribute__((address_space(1))) T*)(uintptr_t)p;
 }
 __device__ __forceinline__ uint32_t rfl(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ uint64_t rfl64(uint64_t v) { return ((uint64_t)rfl((uint32_t)(v >> 32)) << 32) | rfl((uint32_t)v); }
 template <class T>
 __device__ __forceinline__ T* rflp(T* p) {
   const uint64_t a = (uint64_t)(uintptr_t)p;
@@ -484,9 +485,9 @@ __device__ __forceinline__ void pre_prime(const A& ar, const SegJob& J, uint64_t
                                           PreRec* __restrict__ out) {
   using V = typename A::V;
   const JobDev* __restrict__ job = J.job;
-  const uint32_t base = job->base;
-  const int K = job->kcount;
-  const uint32_t algo = job->algo;
+  const uint32_t base = ld_const(&job->base);
+  const int K = ld_const(&job->kcount);
+  const uint32_t algo = ld_const(&job->algo);
   const uint32_t T = J.threads;
   const V vone = ar.one(), nmo = ar.neg_one();
   const V mbase = ar.base(base);
@@ -508,12 +509,12 @@ __device__ __forceinline__ void pre_prime(const A& ar, const SegJob& J, uint64_t
   V P0 = vone, P1 = vone;
   int i = 0;
   for (; i + 1 < K; i += 2) {
-    P0 = ar.mul(P0, ar.from_int(job->ak[i]));
-    P1 = ar.mul(P1, ar.from_int(job->ak[i + 1]));
+    P0 = ar.mul(P0, ar.from_int(ld_const(&job->ak[i])));
+    P1 = ar.mul(P1, ar.from_int(ld_const(&job->ak[i + 1])));
   }
-  if (i < K) P0 = ar.mul(P0, ar.from_int(job->ak[i]));
+  if (i < K) P0 = ar.mul(P0, ar.from_int(ld_const(&job->ak[i])));
   const V P = ar.mul(P0, P1);
-  const V bQ1 = ar.base_pow(mbase, job->Qref - 1, base);  // b^(Qref-1), for b^-1 below
+  const V bQ1 = ar.base_pow(mbase, ld_const(&job->Qref) - 1, base);  // b^(Qref-1), for b^-1 below
   const V gQ = ar.mul(bQ1, mbase);
   const V total = ar.mul(P, gQ);
   const V inv_total = ar.pow_lg(total, p - 2);  // setup.cl:154
@@ -526,7 +527,7 @@ __device__ __forceinline__ void pre_prime(const A& ar, const SegJob& J, uint64_t
   const V binv = ar.mul(bQ1, ar.mul(inv_total, P));  // b^(Qref-1) * b^-Qref (setup.cl:164-165)
   o.v[kPvInvK] = ar.store(ar.mul(inv_total, gQ));
   o.v[kPvBinv] = ar.store(binv);
-  o.v[kPvBn] = ar.store(ar.pow_u64(mbase, job->nmin));
+  o.v[kPvBn] = ar.store(ar.pow_u64(mbase, ld_const(&job->nmin)));
 
   // The power-residue tests b^((p-1)/d) = 1 for d | p - 1 (setup.cl:22-48)
   // and the quadratic character b^((p-1)/2) (setup.cl:51-53).
@@ -617,7 +618,7 @@ __device__ __forceinline__ void pre_prime(const A& ar, const SegJob& J, uint64_t
     // the constants of the reference's tables and of the merged table:
     // b^T and b^2T (baby strides of a workgroup round), b^-Q (giant step)
     // and its T-th and 2T-th powers
-    const uint32_t Q = job->Q;
+    const uint32_t Q = ld_const(&job->Q);
     o.v[kPvBT0] = ar.store(ar.pow_sm(mbase, T));
     o.v[kPvBT1] = ar.store(ar.pow_sm(mbase, 2 * T));
     if (algo == 2) {  // a small prime of a residue-class job: the parity split, table span 2Q
@@ -680,10 +681,10 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
   PrimeRec* __restrict__ R = J.recs + pi;
   Counters* __restrict__ cnt = J.cnt;
   const uint64_t p = pre.p, q = pre.q, one = pre.one;
-  const uint32_t base = job->base;
-  const int K = job->kcount;
+  const uint32_t base = ld_const(&job->base);
+  const int K = ld_const(&job->kcount);
   const uint32_t T = J.threads;
-  const uint32_t algo = job->algo;
+  const uint32_t algo = ld_const(&job->algo);
   const V vone = ar.one(), nmo = ar.neg_one();
   const V mbase = ar.base(base);
   // x^lane in every lane: an inclusive product scan of (lane ? x : 1), six
@@ -699,8 +700,8 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
   };
 
   // k for k = lane and k = lane + 64; 'one' pads the products.
-  const V mk0 = (int)lane < K ? ar.from_int(job->ak[lane]) : vone;
-  const V mk1 = (int)lane + 64 < K ? ar.from_int(job->ak[lane + 64]) : vone;
+  const V mk0 = (int)lane < K ? ar.from_int(ld_global(&job->ak[lane])) : vone;
+  const V mk1 = (int)lane + 64 < K ? ar.from_int(ld_global(&job->ak[lane + 64])) : vone;
 
   // inclusive prefix (s) and suffix (u) products over the wavefront
   V s0 = mk0, s1 = mk1, u0 = mk0, u1 = mk1;
@@ -738,26 +739,27 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
   uint32_t L = 2, La = 2, Lb = 1, Lc = 1, Ld = 1;
   uint64_t exL = 0;
   if (split) {
-    La = pre.La;
-    Lb = pre.Lb;
-    Lc = pre.Lc;
-    Ld = pre.Ld;
+    La = rfl(pre.La);
+    Lb = rfl(pre.Lb);
+    Lc = rfl(pre.Lc);
+    Ld = rfl(pre.Ld);
     L = La * Lb * Lc * Ld;
     ar.divides(pm, L, 1.0 / (double)L, &exL);
+    exL = rfl64(exL);  // wavefront-uniform: the exponentiation loops branch on scalars
   }
 
   // k^-1 and the target hadj for both slots (setup.cl:180-188)
   const V kinv0 = ar.mul(inv_k, ar.mul(pre0, suf0));
   const V kinv1 = ar.mul(inv_k, ar.mul(pre1, suf1));
   const bool act0 = (int)lane < K, act1 = (int)lane + 64 < K;
-  const V h0 = (act0 && job->cpos[lane]) ? ar.mul(nmo, kinv0) : kinv0;
-  const V h1 = (act1 && job->cpos[lane + 64]) ? ar.mul(nmo, kinv1) : kinv1;
+  const V h0 = (act0 && ld_global(&job->cpos[lane])) ? ar.mul(nmo, kinv0) : kinv0;
+  const V h1 = (act1 && ld_global(&job->cpos[lane + 64])) ? ar.mul(nmo, kinv1) : kinv1;
 
   // prefilter (setup.cl:59-87); the order of the tests does not change the
   // result.  h^e for both slots, as +1, -1 or 0 (anything else).
   auto pow_sign2 = [&](uint64_t e, int* t0, int* t1) {
     V a0, a1;
-    ar.pow_lg2(h0, h1, e, &a0, &a1);
+    ar.pow_lg2(h0, h1, rfl64(e), &a0, &a1);  // a wavefront-uniform exponent: scalar loop control
     *t0 = (a0 == vone) - (a0 == nmo);
     *t1 = (a1 == vone) - (a1 == nmo);
   };
@@ -790,7 +792,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     yb1 = ar.pow_u64(u1, La);
   }
   auto sign_of = [&](V x0, V x1, uint32_t e, int* t0, int* t1) {
-    const V a0 = ar.pow_u64(x0, e), a1 = ar.pow_u64(x1, e);
+    const V a0 = ar.pow_u64(x0, rfl(e)), a1 = ar.pow_u64(x1, rfl(e));
     *t0 = (a0 == vone) - (a0 == nmo);
     *t1 = (a1 == vone) - (a1 == nmo);
   };
@@ -990,7 +992,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     uint32_t ra[2] = {0, 0}, rb[2] = {0, 0};
     const int na = classes(cls0, ya0, yb0, yc0, yd0, ra), nb = classes(cls1, ya1, yb1, yc1, yd1, rb);
     const SmallMod MS(S);
-    const uint32_t nmod = job->nmin % S;
+    const uint32_t nmod = ld_const(&job->nmin) % S;
     // b^-d for the shifts d < S < 2^24, in base-64 digits: lane j
     // holds b^-j, b^-(64j), b^-(4096j) and b^-(262144j) (j < 64; the last two
     // levels only when S needs them), so b^-d is one shuffle per digit and a
@@ -1037,19 +1039,20 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     nk[0] = off;
     // Table length: E* = sqrt(span * targets / (2S)) balances the baby
     // steps (set, then matched) against the giant-step probes.  The BSGS
-    // kernel runs giant steps in blocks of 64 (one per lane), so the giant
-    // steps are rounded to a multiple of 64 (E = span / (S*M) rounded up,
-    // then the M that E needs).  E is at most the job's Q entries, and a
-    // capped table takes as many giant steps as it needs.
-    const uint32_t span = job->nmax - job->nmin + 1;
+    // kernel runs giant steps in tile rows of 64 (one per lane), or of 16
+    // or 32 for items with at most 48, so the giant steps are rounded to a
+    // multiple of 64, or of 16 below 56 (E = span / (S*M) rounded up, then
+    // the M that E needs).  E is at most the job's Q entries, and a capped
+    // table takes as many giant steps as it needs.
+    const uint32_t span = ld_const(&job->nmax) - ld_const(&job->nmin) + 1;
     const double estar = sqrt((double)span * (double)(off + 1) / (2.0 * (double)S));
     const double mstar = (double)span / ((double)S * estar);
     // ceil(span / d) in binary64, exact for span < 2^53: a quotient just above
     // an integer n exceeds it by at least 1/d, more than its rounding error
     auto ceil_div = [&](double d) { return (uint64_t)ceil((double)span / d); };
-    const uint64_t M64 = 64 * (uint64_t)fmax(1.0, rint(mstar / 64.0));
-    uint64_t E = ceil_div((double)S * (double)M64);
-    if (E > job->Q) E = job->Q;
+    const uint64_t Mrow = mstar < 56.0 ? 16 * (uint64_t)fmax(1.0, rint(mstar / 16.0)) : 64 * (uint64_t)rint(mstar / 64.0);
+    uint64_t E = ceil_div((double)S * (double)Mrow);
+    if (E > ld_const(&job->Q)) E = ld_const(&job->Q);
     const uint64_t M = ceil_div((double)S * (double)E);
     const uint64_t SE = (uint64_t)S * E;
     cS = S;
@@ -1136,8 +1139,8 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
       nk[0] = off;
       if (algo == 2) {  // a small p of a residue-class job: the parity split (S = 2)
         cS = 2;
-        cE = job->Q;
-        cM = (uint32_t)(((uint64_t)job->nmax - job->nmin + 1 + 2ull * cE - 1) / (2ull * cE));
+        cE = ld_const(&job->Q);
+        cM = (uint32_t)(((uint64_t)ld_const(&job->nmax) - ld_const(&job->nmin) + 1 + 2ull * cE - 1) / (2ull * cE));
         // (g0v and gT0v are b^-2Q and b^-2QT here, staged for this case)
         bTb = ar.mul(mbase, mbase);  // the table stride b^S, S = 2
       }
@@ -1460,7 +1463,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
   __shared__ uint64_t s_pw[2][kLgT];  // per-item power tables (see vstart)
   extern __shared__ uint4 lds4[];
   uint32_t* tab = (uint32_t*)lds4;  // the filter; the bucketed table of the overflow recovery reuses it
-  __shared__ uint64_t s_h[2 * RS_MAXK + 16];  // targets of the item (+ read-ahead pad)
+  __shared__ uint64_t s_h[2 * RS_MAXK + 24];  // targets of the item (+ the last tile's pad, rg*kU <= 24)
   static_assert(RS_MAXK <= 256, "a sequence index fits the low 8 bits of s_k");
   __shared__ uint32_t s_k[2 * RS_MAXK];   // sequence index | shift (< 2^21) << 8
   __shared__ uint32_t s_item, s_order, s_dups, s_claims, s_qn;
@@ -1632,17 +1635,21 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       const uint32_t slot = atomicAdd(&s_qn, 1u);
       if (slot < qcap) s_q[slot] = make_uint4((uint32_t)g, (uint32_t)(g >> 32), i, qi);
     };
-    // Giant steps in tiles: a tile is 64 consecutive giant steps, one per
-    // lane (q = 64*qb + lane), times kU consecutive targets (group g); tile
+    // Giant steps in tiles: a tile is a row of rw consecutive giant steps
+    // (rw = 64, or 16 or 32 for an item with few giant steps) times rg =
+    // 64 / rw groups of kU consecutive targets: lane l takes giant step
+    // q = rw*qb + (l mod rw) and the targets of group g*rg + l / rw.  Tile
     // t = qb*ng + g, and wavefront w of the workgroup takes the tiles w,
     // w + kW, w + 2 kW, ... (step j holds the tiles j*kW .. j*kW + kW - 1).
     // Every probe of the item, whatever M and the target count, then runs
-    // through the one software-pipelined loop below, and the wavefronts get
-    // equal shares (lanes with q >= M and targets past nk are computed and
-    // ignored).
+    // through the one loop below, and the wavefronts get equal shares (lanes
+    // with q >= M and targets past nk are computed and ignored).
     // (the wavefront index through readfirstlane: the tile walk is scalar)
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6)), lane = tid & 63;
-    const uint32_t ng = (nk + kU - 1) / kU, nblk = (Ml + 63) / 64, ntiles = nblk * ng;
+    // row width: the fewest idle lanes, rows of 64 from 49 giant steps on
+    const uint32_t lw = Ml <= 16 ? 4u : Ml <= 32 ? 5u : Ml <= 48 ? 4u : 6u;
+    const uint32_t rw = 1u << lw, rg = 64u >> lw, lsub = lane & (rw - 1), lgrp = lane >> lw;
+    const uint32_t ng = (nk + rg * kU - 1) / (rg * kU), nblk = (Ml + rw - 1) / rw, ntiles = nblk * ng;
     const uint32_t nsteps = (ntiles + kW - 1) / kW;
     // Steps between queue checks: a step queues about T * kU * 5e-4
     // positives (the filter's rate at half load); checking when a quarter of
@@ -1873,7 +1880,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       };
       const uint64_t G = Gm;
       for (uint32_t t = ja * kW + wave; t < ntiles; t += kW) {
-        const uint32_t qi = (t / ng) * 64 + lane, g0 = (t % ng) * kU;
+        const uint32_t qi = (t / ng) * rw + lsub, g0 = ((t % ng) * rg + lgrp) * kU;
         if (qi >= Ml) continue;
         const uint64_t Gq = powmod_sm(G, qi, p, q, one);
         for (uint32_t i = g0; i < g0 + kU && i < nk; i++) resolve(mmul(kr[i].hadj, Gq, p, q), i, qi);
@@ -2056,32 +2063,32 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         using A = typename std::decay<decltype(ar)>::type;
         using Tg = typename A::Tgt;
         using Pw = typename A::Pow;
-        // this wavefront's first tile and its giant power G^(64*qb + lane):
-        // G^lane from s_gl times the power table's entries for the bits of
-        // the wavefront-uniform qb
+        // this wavefront's first tile and its giant power G^(rw*qb + lsub):
+        // G^lsub from s_gl times the power table's entries for the bits of
+        // the wavefront-uniform rw*qb (below T)
         uint32_t qb = wave / ng, g = wave % ng;
-        Pw gq = ar.power(s_gl[lane]);
+        Pw gq = ar.power(s_gl[lsub]);
 #pragma unroll
-        for (int i = 6; i < kLgT; i++)
-          if ((qb >> (i - 6)) & 1u) gq = ar.mul(gq, ar.power(s_pw[1][i]));
-        const Pw g64 = ar.power(s_pw[1][6]);  // G^64: one tile row further
+        for (int i = 4; i < kLgT; i++)
+          if (i >= (int)lw && ((qb >> (i - lw)) & 1u)) gq = ar.mul(gq, ar.power(s_pw[1][i]));
+        const Pw grw = ar.power(s_pw[1][lw]);  // G^rw: one tile row further
         Tg tt[kU];
         auto load_t = [&]() {
 #pragma unroll
-          for (uint32_t k = 0; k < kU; k++) tt[k] = ar.target(s_h[g * kU + k]);
+          for (uint32_t k = 0; k < kU; k++) tt[k] = ar.target(s_h[(g * rg + lgrp) * kU + k]);
         };
         // products of the current tile (targets in tt), then advance to the
         // wavefront's next tile
         auto mul_t = [&](uint64_t(&uu)[kU], uint32_t& i0, uint32_t& qbt) {
 #pragma unroll
           for (uint32_t k = 0; k < kU; k++) uu[k] = ar.key(tt[k], gq);
-          i0 = g * kU;
+          i0 = (g * rg + lgrp) * kU;
           qbt = qb;
           g = (uint32_t)__builtin_amdgcn_readfirstlane((int)(g + kW));
           while (g >= ng) {
             g -= ng;
             qb++;
-            gq = ar.mul(gq, g64);
+            gq = ar.mul(gq, grw);
           }
         };
         auto read_w = [&](const uint64_t(&uu)[kU], uint32_t(&ww)[kU]) {
@@ -2095,7 +2102,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
             x[k] = ~ww[k] & flt_bits(flt, uu[k]);
             mn = min(mn, x[k]);
           }
-          const uint32_t qq = qbt * 64 + lane;
+          const uint32_t qq = qbt * rw + lsub;
           if (mn == 0u && qq < Ml) {
 #pragma unroll
             for (uint32_t k = 0; k < kU; k++)
