@@ -316,14 +316,33 @@ __constant__ uint32_t g_prd[12] = {3, 4, 5, 7, 8, 9, 11, 13, 17, 19, 23, 29};
 // searched, and the classes only narrow the n they are searched on, so the
 // solutions found are the same.  For a prime p with b a quadratic
 // non-residue s is even and fixes the parity the reference's class gives.
-// A fourth part Ld, one prime r in [17, 127] that divides p - 1, with zd =
-// b^((p-1)/Ld) != 1 (so of order r for a prime p) and yd = h^((p-1)/Ld),
-// narrows n further mod r (the largest such r whose stride stays below 2^24,
-// the shifts' width in the BSGS queue; 1 when none): L = La*Lb*Lc*Ld.
-constexpr uint32_t kRd[25] = {17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67,
-                              71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113, 127};
-__constant__ uint32_t g_rd[25] = {17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67,
-                                  71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113, 127};
+// A fourth part Ld, one prime r in [17, 1021] that divides p - 1, with zd =
+// b^((p-1)/Ld) != 1 and zd^Ld = 1 (of order r, always so for a prime p) and
+// yd = h^((p-1)/Ld), narrows n further mod r (the largest such r whose stride
+// stays below 2^24, the shifts' width in the BSGS queue; 1 when none):
+// L = La*Lb*Lc*Ld.  Its discrete logarithm is a small baby-step giant-step:
+// at most 128 powers zd^i in the table and ceil(Ld / 128) giant steps.
+constexpr int kNRd = 166;
+constexpr uint32_t kRd[kNRd] = {
+    17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113,
+    127, 131, 137, 139, 149, 151, 157, 163, 167, 173, 179, 181, 191, 193, 197, 199, 211, 223, 227, 229,
+    233, 239, 241, 251, 257, 263, 269, 271, 277, 281, 283, 293, 307, 311, 313, 317, 331, 337, 347, 349,
+    353, 359, 367, 373, 379, 383, 389, 397, 401, 409, 419, 421, 431, 433, 439, 443, 449, 457, 461, 463,
+    467, 479, 487, 491, 499, 503, 509, 521, 523, 541, 547, 557, 563, 569, 571, 577, 587, 593, 599, 601,
+    607, 613, 617, 619, 631, 641, 643, 647, 653, 659, 661, 673, 677, 683, 691, 701, 709, 719, 727, 733,
+    739, 743, 751, 757, 761, 769, 773, 787, 797, 809, 811, 821, 823, 827, 829, 839, 853, 857, 859, 863,
+    877, 881, 883, 887, 907, 911, 919, 929, 937, 941, 947, 953, 967, 971, 977, 983, 991, 997, 1009, 1013,
+    1019, 1021};
+__constant__ uint32_t g_rd[kNRd] = {
+    17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113,
+    127, 131, 137, 139, 149, 151, 157, 163, 167, 173, 179, 181, 191, 193, 197, 199, 211, 223, 227, 229,
+    233, 239, 241, 251, 257, 263, 269, 271, 277, 281, 283, 293, 307, 311, 313, 317, 331, 337, 347, 349,
+    353, 359, 367, 373, 379, 383, 389, 397, 401, 409, 419, 421, 431, 433, 439, 443, 449, 457, 461, 463,
+    467, 479, 487, 491, 499, 503, 509, 521, 523, 541, 547, 557, 563, 569, 571, 577, 587, 593, 599, 601,
+    607, 613, 617, 619, 631, 641, 643, 647, 653, 659, 661, 673, 677, 683, 691, 701, 709, 719, 727, 733,
+    739, 743, 751, 757, 761, 769, 773, 787, 797, 809, 811, 821, 823, 827, 829, 839, 853, 857, 859, 863,
+    877, 881, 883, 887, 907, 911, 919, 929, 937, 941, 947, 953, 967, 971, 977, 983, 991, 997, 1009, 1013,
+    1019, 1021};
 constexpr uint32_t kZV = 464;                 // za^i at [0, 81), zb^i at [128, 192), zc^i at [192, 336),
                                               // zd^i at [336, 464)
 constexpr uint32_t kZH = 896;                 // their hash slots: [0, 256), [256, 384), [384, 640), [640, 896)
@@ -346,7 +365,7 @@ struct SmallMod {
   }
 };
 
-// a^-1 mod m for gcd(a, m) = 1 and m <= 192, by every lane of the wavefront
+// a^-1 mod m for gcd(a, m) = 1 and m < 2^11, by every lane of the wavefront
 // trying its candidates at once (call with all lanes active); 0 for m = 1
 __device__ __forceinline__ uint32_t inv_mod_lanes(uint32_t a, const SmallMod& M, uint32_t lane) {
   const uint32_t am = M(a);
@@ -406,6 +425,7 @@ struct MontArith {
   __device__ uint64_t bits(V x) const { return x; }  // a canonical integer key
   __device__ uint64_t store(V x) const { return x; }  // a value as staged between the setup kernels
   __device__ V load(uint64_t u) const { return u; }
+  __device__ V from_bits(uint64_t u) const { return u; }  // the inverse of bits()
 };
 
 struct FpArith {
@@ -440,6 +460,7 @@ struct FpArith {
   __device__ uint64_t bits(V x) const { return fp_to_u64(x); }
   __device__ uint64_t store(V x) const { return (uint64_t)__double_as_longlong(x); }
   __device__ V load(uint64_t u) const { return __longlong_as_double((long long)u); }
+  __device__ V from_bits(uint64_t u) const { return (double)u; }  // the inverse of bits() (u < 2^51)
 };
 
 // ---------------------------------------------------------------------------
@@ -538,26 +559,27 @@ __device__ __forceinline__ void pre_prime(const A& ar, const SegJob& J, uint64_t
   if (p > kClassMinP) {
     // With L = gcd(p - 1, 720720) and z = b^((p-1)/L), b^((p-1)/d) = z^(L/d)
     // for every d of the tests that divides 720720 (d | p - 1 <=> d | L);
-    // for 17, 19, 23 and 29, w = b^((p-1)/D) with D the product of those
-    // that divide p - 1 gives b^((p-1)/d) = w^(D/d).  The same residues as
-    // the direct exponents, in any ring: (p-1)/L and (p-1)/D are at least 4
-    // here, where pow_lg's masked shift is exact.
+    // for the primes r of kRd (17, 19, 23 and 29 among them), w = b^((p-1)/D)
+    // with D the product of those that divide p - 1 gives b^((p-1)/r) =
+    // w^(D/r).  The same residues as the direct exponents, in any ring:
+    // (p-1)/L is at least 2 here (L <= 720720), and so is (p-1)/D (D is odd),
+    // where pow_lg's masked shift is exact.
     uint32_t La, Lb, Lc;
     class_moduli(ar, pm, &La, &Lb, &Lc);
     const uint32_t L = La * Lb * Lc;
     uint64_t exL;
     ar.divides(pm, L, 1.0 / (double)L, &exL);
     const V z = ar.pow_lg(mbase, exL);
-    // D: the primes r of g_rd that divide p - 1 (17, 19, 23 and 29 first,
+    // D: the primes r of kRd that divide p - 1 (17, 19, 23 and 29 first,
     // the reference's tests; the others while D stays below 2^32)
     uint32_t D = 1;
-    uint32_t dm = 0;  // bit j: kRd[j] divides D
+    uint32_t dm[(kNRd + 31) / 32] = {};  // bit j % 32 of dm[j / 32]: kRd[j] divides D
 #pragma unroll
-    for (int j = 0; j < 25; j++) {
+    for (int j = 0; j < kNRd; j++) {
       uint64_t qt;
       if (ar.divides(pm, kRd[j], 1.0 / kRd[j], &qt) && (j < 4 || (uint64_t)D * kRd[j] < (1ull << 32))) {
         D *= kRd[j];
-        dm |= 1u << j;
+        dm[j >> 5] |= 1u << (j & 31);
       }
     }
     V w = vone;
@@ -574,15 +596,18 @@ __device__ __forceinline__ void pre_prime(const A& ar, const SegJob& J, uint64_t
     // class part (the largest r with b^((p-1)/r) != 1 and 2*L*r < 2^24)
     uint32_t Ld = 1;
     V zd = vone;
-    for (uint32_t m = dm; m; m &= m - 1) {
-      const int j = __ffs((int)m) - 1;
-      const uint32_t r = g_rd[j];
-      const V t = ar.pow_u64(w, D / r);
-      if (t == vone) {
-        if (j < 4) dmask |= 1u << (8 + j);
-      } else if (2ull * L * r < (1ull << 24)) {
-        Ld = r;
-        zd = t;
+#pragma unroll
+    for (int wi = 0; wi < (kNRd + 31) / 32; wi++) {
+      for (uint32_t m = dm[wi]; m; m &= m - 1) {
+        const int j = 32 * wi + __ffs((int)m) - 1;
+        const uint32_t r = g_rd[j];
+        const V t = ar.pow_u64(w, D / r);
+        if (t == vone) {
+          if (j < 4) dmask |= 1u << (8 + j);
+        } else if (2ull * L * r < (1ull << 24)) {
+          Ld = r;
+          zd = t;
+        }
       }
     }
     rg = ar.pow_u64(z, L / 2);
@@ -770,8 +795,10 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
   // them from there.
   V ya0 = vone, ya1 = vone, yb0 = vone, yb1 = vone, yc0 = vone, yc1 = vone, yd0 = vone, yd1 = vone;
   if (split) {
-    V t0, t1;
-    ar.pow_lg2(h0, h1, exL, &t0, &t1);  // y
+    // y = h^((p-1)/L); the exponent is 1 when L = p - 1 (pow_lg's masked
+    // shift would take h^(2^64+1) there)
+    V t0 = h0, t1 = h1;
+    if (exL > 1) ar.pow_lg2(h0, h1, exL, &t0, &t1);
     if (Ld > 1) {
       yd0 = ar.pow_u64(t0, La * Lb * Lc);
       yd1 = ar.pow_u64(t1, La * Lb * Lc);
@@ -885,7 +912,20 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     // (Lc = 1 for most p: then sc = 1 and the third table is not used; Ld = 1
     // likewise for the fourth)
     const uint32_t sa = table(za, La, 0u), sb = table(zb, Lb, 128u), sc = Lc > 1 ? table(zc, Lc, 192u) : 1u;
-    const uint32_t sd = Ld > 1 ? table(zd, Ld, 336u) : 1u;
+    // the fourth part: the baby steps zd^i, i < Bd = min(Ld, 128)
+    const uint32_t Bd = Ld < 128 ? Ld : 128;
+    if (Ld > 1) table(zd, Bd - 1, 336u);
+    wave_sync();
+    // sd = Ld when zd^Ld = 1 (zd != 1 and Ld prime: the order of zd), else 0;
+    // gz = zd^-Bd = zd^(Ld - Bd), the giant step of its discrete logarithm
+    uint32_t sd = 1;
+    V gz = vone;
+    if (Ld > 1) {
+      const V zB = ar.mul(ar.from_bits(zv[336 + Bd - 1]), zd);  // zd^Bd
+      const uint32_t qd = Ld / Bd, rd = Ld - qd * Bd;
+      if (qd > 1 || rd) gz = ar.mul(ar.pow_u64(zB, qd - 1), rd ? ar.from_bits(zv[336 + rd]) : vone);
+      sd = ar.mul(gz, zB) == vone ? Ld : 0u;
+    }
     // The CRT needs pairwise coprime orders.  For a prime p they divide 80,
     // 63, 143 and the prime Ld and are coprime; a composite p may give other
     // orders, which are then tested by Euclid.
@@ -902,7 +942,6 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
                          ((80u % sa == 0 && 63u % sb == 0 && 143u % sc == 0 && (sd == 1 || sd == Ld)) ||
                           (gcd(sa, sb) == 1 && gcd(sab, sc) == 1 && gcd(sabc, sd) == 1));
     const uint32_t s = coprime ? sabc * sd : 0u;
-    wave_sync();
     if (s) {
       for (uint32_t idx = lane; idx < sa; idx += 64) {
         uint32_t h = (uint32_t)zv[idx] & 255u;
@@ -916,7 +955,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
         uint32_t h = (uint32_t)zv[192 + idx] & 255u;
         while (atomicCAS(&zh[384 + h], 0u, idx + 1) != 0u) h = (h + 1) & 255u;
       }
-      for (uint32_t idx = lane; Ld > 1 && idx < sd; idx += 64) {
+      for (uint32_t idx = lane; Ld > 1 && idx < Bd; idx += 64) {
         uint32_t h = (uint32_t)zv[336 + idx] & 255u;
         while (atomicCAS(&zh[640 + h], 0u, idx + 1) != 0u) h = (h + 1) & 255u;
       }
@@ -946,8 +985,17 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
         xc = lookup(ar.bits(ycv), 384u, 255u, 192u);
         if (xc < 0) return -1;
       }
-      if (Ld > 1) {
-        xd = lookup(ar.bits(ydv), 640u, 255u, 336u);
+      if (Ld > 1) {  // the first j with yd * zd^(-Bd*j) = zd^i, i < Bd: xd = i + Bd*j < Ld
+        V g = ydv;
+        xd = -1;
+        for (uint32_t j = 0; j * Bd < Ld; j++) {
+          const int i = lookup(ar.bits(g), 640u, 255u, 336u);
+          if (i >= 0) {
+            xd = i + (int)(Bd * j);
+            break;
+          }
+          g = ar.mul(g, gz);
+        }
         if (xd < 0) return -1;
       }
       const uint32_t dx = Mb((uint32_t)xb + sb - Mb((uint32_t)xa));

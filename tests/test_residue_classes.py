@@ -4,8 +4,9 @@ checked by brute force.
 For L = gcd(p - 1, 720720) * Ld and z = b^((p-1)/L), every n with b^n = h
 (mod p) satisfies z^n = y with y = h^((p-1)/L), so n = x (mod s) for the
 x < s = ord(z) with z^x = y, and there is no n when y is not a power of z.
-Ld is one prime r in [17, 127] dividing p - 1 with b^((p-1)/r) != 1 (the
-largest whose stride 2 * gcd(p - 1, 720720) * r stays below 2^24), or 1.
+Ld is one prime r in [17, 1021] dividing p - 1 with b^((p-1)/r) != 1 (the
+largest whose stride 2 * gcd(p - 1, 720720) * r stays below 2^24), or 1; its
+order is Ld when (b^((p-1)/Ld))^Ld = 1, else the rule keeps the parity split.
 The setup keeps the reference's own tests (setup.cl:59-87: the parity classes and the
 power-residue tests) and only narrows the n searched to that class, so the
 solutions found in [nmin, nmax] must be exactly those of the reference's
@@ -20,7 +21,7 @@ import random
 import pytest
 
 PRD = [3, 4, 5, 7, 8, 9, 11, 13, 17, 19, 23, 29]  # setup.cl:22-48
-RD = [17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113, 127]
+RD = [r for r in range(17, 1022) if all(r % q for q in range(2, math.isqrt(r) + 1))]
 
 
 def _sprp2(n):
@@ -92,7 +93,7 @@ def _class_residues(p, b, hs, clss):
     sb = next((i for i in range(1, Lb + 1) if pow(zb, i, p) == 1), 0)
     # Lc = 1: no constraint mod sc = 1 and no third table (as the kernel); Ld likewise
     sc = 1 if Lc == 1 else next((i for i in range(1, Lc + 1) if pow(zc, i, p) == 1), 0)
-    sd = 1 if Ld == 1 else next((i for i in range(1, Ld + 1) if pow(zd, i, p) == 1), 0)
+    sd = 1 if Ld == 1 else (Ld if pow(zd, Ld, p) == 1 else 0)
     coprime = math.gcd(sa, sb) == 1 and math.gcd(sa * sb, sc) == 1 and math.gcd(sa * sb * sc, sd) == 1
     s = sa * sb * sc * sd if sa and sb and sc and sd and coprime else 0
     any_par = any(c >= 2 for c in clss)
