@@ -593,6 +593,19 @@ def main():
         "factors_verified": int(res["verified"][1]),
         "setup_s": setup_s,
     }
+    if tuned and not args.tuned_only:
+        # The reference algorithm's work on the same primes: the plain leg
+        # runs the reference split (three tables, giant.cl's loop bounds) and
+        # counts its modular products exactly (the oracle's count, checked in
+        # the parity tests).  At the tuned leg's rate that work would need
+        # this much FP64 throughput: the effect of the residue-class tables,
+        # shown beside the executed-work roofline above.
+        ref_pp = plain["roof"]["modmul_per_launch"] * plain["roof"]["launches"] / max(plain["primes"], 1)
+        ach = ref_pp * value * FLOPS_PER_MODMUL / 1e12
+        line["roofline"]["reference_equivalent"] = {
+            "modmul_per_prime_reference": ref_pp,
+            "modmul_per_prime_executed": res["roof"]["modmul_per_launch"] * res["roof"]["launches"] / max(res["primes"], 1),
+            "achieved": ach, "unit": "TFLOP/s", "frac": ach / (FP64_PEAK / 1e12)}
     if tuned:
         line["tuned_log"] = tuned["log"]
     if jobs16 is not None:
