@@ -31,15 +31,15 @@ struct JobDev {
 // Per-prime record written by the setup kernel (128 bytes).
 struct PrimeRec {
   uint64_t p, q, one, mbase;
-  uint64_t bn;                // b^NMIN (Montgomery)
-  uint64_t g[2];              // giant step: b^-Q (full) and b^-QQ (parity)
+  uint64_t bn;                // b^NMIN (Montgomery); residue-class tables: b^-NMIN
+  uint64_t g[2];              // giant step: b^-Q (full) and b^-QQ (parity); residue classes: b^(S*E)
   uint64_t gT[2];             // g[c]^threads
   uint64_t bT[2];             // baby stride per round b^(stride*threads) (bT[0]; three lists: b^threads,
-                              // b^(2*threads)); residue classes: bT[1] = b^S
+                              // b^(2*threads)); residue classes: bT[0] = b^-(S*threads), bT[1] = b^-S
   uint32_t nk[RS_NCLASS];     // k count per class: full, even, odd (0 = skipped)
   uint32_t flags;             // the job's algo when written
-  // residue-class tables (algo 2): baby values b^(NMIN + S*e), e < E; M giant
-  // steps of b^(-S*E); every target carries its n mod S as a shift
+  // residue-class tables (algo 2): baby values b^-(NMIN + S*e), e < E; M giant
+  // steps of b^(S*E); every target carries its n mod S as a shift
   uint32_t S, E, M, pad32;
   uint64_t pad;
 };
@@ -47,7 +47,8 @@ static_assert(sizeof(PrimeRec) == 128, "PrimeRec layout");
 
 // Per (prime, k) entry, grouped by class: full, then even, then odd.
 struct KRec {
-  uint64_t hadj;              // target b^n: k^-1 (c=-1) or -k^-1 (c=+1), Montgomery (times b^-d)
+  uint64_t hadj;              // target b^n: k^-1 (c=-1) or -k^-1 (c=+1), Montgomery (merged tables: times
+                              // b^-d); residue-class tables: the inverse times b^d, -c*k*b^d
   uint32_t kidx;              // sequence index
   uint32_t cls;               // three lists: 1 full, 2 even, 3 odd (the reference's parity code);
                               // merged and residue-class tables: the shift d of the target

@@ -484,9 +484,10 @@ struct PreRec {
 static_assert(sizeof(PreRec) == sizeof(PrimeRec), "PreRec fills a PrimeRec slot");
 static_assert(offsetof(PreRec, nk) == offsetof(PrimeRec, nk), "a skipped prime's staged nk is its final nk");
 constexpr uint32_t kPreSkip = 1u;
-// slots of PreRec::v: (prod of the k)^-1, b^-1, b^NMIN; then for residue
-// classes za, zb, zc, zd, otherwise b^-Q, b^-QT, b^-2QT, b^T, b^2T (a small
-// prime of a residue-class job: b^-2Q, b^-2QT for its parity split)
+// slots of PreRec::v: (prod of the k)^-1, b^-1, the first baby value (b^NMIN,
+// b^-NMIN for residue classes); then for residue classes za, zb, zc, zd,
+// otherwise b^-Q, b^-QT, b^-2QT, b^T, b^2T (a small prime of a residue-class
+// job: b^-2Q, b^-2QT for its parity split)
 enum { kPvInvK = 0, kPvBinv = 1, kPvBn = 2, kPvZa = 3, kPvZb = 4, kPvZc = 5, kPvZd = 6,
        kPvG0 = 3, kPvGT0 = 4, kPvGT1 = 5, kPvBT0 = 6, kPvBT1 = 7 };
 
@@ -548,14 +549,15 @@ __device__ __forceinline__ void pre_prime(const A& ar, const SegJob& J, uint64_t
   const V binv = ar.mul(bQ1, ar.mul(inv_total, P));  // b^(Qref-1) * b^-Qref (setup.cl:164-165)
   o.v[kPvInvK] = ar.store(ar.mul(inv_total, gQ));
   o.v[kPvBinv] = ar.store(binv);
-  o.v[kPvBn] = ar.store(ar.pow_u64(mbase, ld_const(&job->nmin)));
+  // the first baby value: b^NMIN, or b^-NMIN for a residue-class table (which walks b^-n)
+  const bool split = algo == 2 && p > kClassMinP;
+  o.v[kPvBn] = ar.store(ar.pow_u64(split ? binv : mbase, ld_const(&job->nmin)));
 
   // The power-residue tests b^((p-1)/d) = 1 for d | p - 1 (setup.cl:22-48)
   // and the quadratic character b^((p-1)/2) (setup.cl:51-53).
   const uint64_t pm = p - 1;
   uint32_t dmask = 0;
   V rg;
-  const bool split = algo == 2 && p > kClassMinP;
   if (p > kClassMinP) {
     // With L = gcd(p - 1, 720720) and z = b^((p-1)/L), b^((p-1)/d) = z^(L/d)
     // for every d of the tests that divides 720720 (d | p - 1 <=> d | L);
@@ -727,38 +729,16 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
   // k for k = lane and k = lane + 64; 'one' pads the products.
   const V mk0 = (int)lane < K ? ar.from_int(ld_global(&job->ak[lane])) : vone;
   const V mk1 = (int)lane + 64 < K ? ar.from_int(ld_global(&job->ak[lane + 64])) : vone;
-
-  // inclusive prefix (s) and suffix (u) products over the wavefront
-  V s0 = mk0, s1 = mk1, u0 = mk0, u1 = mk1;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const V a0 = shfl_up64(s0, d), a1 = shfl_up64(s1, d);
-    const V b0 = shfl_dn64(u0, d), b1 = shfl_dn64(u1, d);
-    if ((int)lane >= d) {
-      s0 = ar.mul(s0, a0);
-      s1 = ar.mul(s1, a1);
-    }
-    if ((int)lane + d < 64) {
-      u0 = ar.mul(u0, b0);
-      u1 = ar.mul(u1, b1);
-    }
-  }
-  const V A0 = shfl64(s0, 63), B0 = shfl64(s1, 63);
-  const V ep0 = shfl_up64(s0, 1), ep1 = shfl_up64(s1, 1);
-  const V es0 = shfl_dn64(u0, 1), es1 = shfl_dn64(u1, 1);
-  // products of all other k: before and after, for both slots of this lane
-  const V pre0 = lane ? ep0 : vone;
-  const V pre1 = ar.mul(A0, lane ? ep1 : vone);
-  const V suf0 = ar.mul(lane < 63 ? es0 : vone, B0);
-  const V suf1 = lane < 63 ? es1 : vone;
+  const bool act0 = (int)lane < K, act1 = (int)lane + 64 < K;
+  const bool cp0 = act0 && ld_global(&job->cpos[lane]), cp1 = act1 && ld_global(&job->cpos[lane + 64]);
 
   // the values of k_setup_pre: the prime passed the invertibility test
-  const V inv_k = ar.load(pre.v[kPvInvK]);  // (prod of all k)^-1
   const V binv = ar.load(pre.v[kPvBinv]);   // b^-1
   const uint64_t dmask = (pre.info >> 1) & 0xFFFu;  // passing d tests, bit i = g_prd[i]
   const int resg = (int)((pre.info >> 13) & 3u) - 1;
 
-  // residue classes: L = La * Lb * Lc = gcd(p - 1, 80) * gcd(p - 1, 63) * gcd(p - 1, 143)
+  // residue classes: L = La * Lb * Lc * Ld (gcd(p - 1, 80), gcd(p - 1, 63),
+  // gcd(p - 1, 143) and the fourth part)
   const uint64_t pm = p - 1;
   const bool split = algo == 2 && p > kClassMinP;
   uint32_t L = 2, La = 2, Lb = 1, Lc = 1, Ld = 1;
@@ -773,12 +753,49 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     exL = rfl64(exL);  // wavefront-uniform: the exponentiation loops branch on scalars
   }
 
-  // k^-1 and the target hadj for both slots (setup.cl:180-188)
-  const V kinv0 = ar.mul(inv_k, ar.mul(pre0, suf0));
-  const V kinv1 = ar.mul(inv_k, ar.mul(pre1, suf1));
-  const bool act0 = (int)lane < K, act1 = (int)lane + 64 < K;
-  const V h0 = (act0 && ld_global(&job->cpos[lane])) ? ar.mul(nmo, kinv0) : kinv0;
-  const V h1 = (act1 && ld_global(&job->cpos[lane + 64])) ? ar.mul(nmo, kinv1) : kinv1;
+  // The target h of a k (setup.cl:180-188): k^-1 for c = -1, -k^-1 for
+  // c = +1.  A residue-class table runs on the inverse instead: its baby
+  // steps walk b^-(nstart + S*e) and its giant steps multiply by b^(S*E), so
+  // a solution b^n = h, n = nstart + d + S*(e + E*q), meets at
+  // h^-1 * b^d * b^(S*E*q) = b^-(nstart + S*e), and h^-1 = -c*k needs no
+  // inversion.  The characters of h and h^-1 agree (a power of h is 1 or -1
+  // exactly when the same power of h^-1 is), and the class of h^-1 is the
+  // negated class of h.  Below, h0 and h1 are h^-1 on that path and h on
+  // the others.
+  V h0, h1;
+  if (split) {
+    h0 = cp0 ? ar.mul(nmo, mk0) : mk0;
+    h1 = cp1 ? ar.mul(nmo, mk1) : mk1;
+  } else {
+    // batch inversion by inclusive prefix (s) and suffix (u) products over the wavefront
+    V s0 = mk0, s1 = mk1, u0 = mk0, u1 = mk1;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const V a0 = shfl_up64(s0, d), a1 = shfl_up64(s1, d);
+      const V b0 = shfl_dn64(u0, d), b1 = shfl_dn64(u1, d);
+      if ((int)lane >= d) {
+        s0 = ar.mul(s0, a0);
+        s1 = ar.mul(s1, a1);
+      }
+      if ((int)lane + d < 64) {
+        u0 = ar.mul(u0, b0);
+        u1 = ar.mul(u1, b1);
+      }
+    }
+    const V A0 = shfl64(s0, 63), B0 = shfl64(s1, 63);
+    const V ep0 = shfl_up64(s0, 1), ep1 = shfl_up64(s1, 1);
+    const V es0 = shfl_dn64(u0, 1), es1 = shfl_dn64(u1, 1);
+    // products of all other k: before and after, for both slots of this lane
+    const V pre0 = lane ? ep0 : vone;
+    const V pre1 = ar.mul(A0, lane ? ep1 : vone);
+    const V suf0 = ar.mul(lane < 63 ? es0 : vone, B0);
+    const V suf1 = lane < 63 ? es1 : vone;
+    const V inv_k = ar.load(pre.v[kPvInvK]);  // (prod of all k)^-1
+    const V kinv0 = ar.mul(inv_k, ar.mul(pre0, suf0));
+    const V kinv1 = ar.mul(inv_k, ar.mul(pre1, suf1));
+    h0 = cp0 ? ar.mul(nmo, kinv0) : kinv0;
+    h1 = cp1 ? ar.mul(nmo, kinv1) : kinv1;
+  }
 
   // prefilter (setup.cl:59-87); the order of the tests does not change the
   // result.  h^e for both slots, as +1, -1 or 0 (anything else).
@@ -1021,9 +1038,9 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
         rr[0] = (uint32_t)(cls - 2);
         return 1;
       }
-      const int xl = dlog(yav, ybv, ycv, ydv);
+      const int xl = dlog(yav, ybv, ycv, ydv);  // the class of h^-1
       if (xl < 0) return 0;
-      const uint32_t ux = (uint32_t)xl;
+      const uint32_t ux = xl ? s - (uint32_t)xl : 0u;  // the class of h
       if (S == s) {
         if (cls >= 2 && (ux & 1u) != (uint32_t)(cls - 2)) return 0;  // s even: the parity contradicts
         rr[0] = ux;
@@ -1041,20 +1058,20 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     const int na = classes(cls0, ya0, yb0, yc0, yd0, ra), nb = classes(cls1, ya1, yb1, yc1, yd1, rb);
     const SmallMod MS(S);
     const uint32_t nmod = ld_const(&job->nmin) % S;
-    // b^-d for the shifts d < S < 2^24, in base-64 digits: lane j
-    // holds b^-j, b^-(64j), b^-(4096j) and b^-(262144j) (j < 64; the last two
-    // levels only when S needs them), so b^-d is one shuffle per digit and a
-    // product between (taken with every lane active, outside the divergent
-    // stores)
-    const V bl = lane_pow(binv);
-    const V b64 = ar.mul(shfl64(bl, 63), binv);  // b^-64
+    // b^d for the shifts d < S < 2^24 (the target of h^-1 is h^-1 * b^d), in
+    // base-64 digits: lane j holds b^j, b^(64j), b^(4096j) and b^(262144j)
+    // (j < 64; the last two levels only when S needs them), so b^d is one
+    // shuffle per digit and a product between (taken with every lane active,
+    // outside the divergent stores)
+    const V bl = lane_pow(mbase);
+    const V b64 = ar.mul(shfl64(bl, 63), mbase);  // b^64
     const V bh = lane_pow(b64);
     V b3 = vone, b4 = vone;
     if (S > 4096) {
-      b3 = lane_pow(ar.mul(shfl64(bh, 63), b64));  // (b^-4096)^j
+      b3 = lane_pow(ar.mul(shfl64(bh, 63), b64));  // (b^4096)^j
       if (S > 262144) b4 = lane_pow(ar.mul(shfl64(b3, 63), ar.mul(shfl64(bh, 63), b64)));
     }
-    auto binv_pow = [&](uint32_t d) {
+    auto bdig_pow = [&](uint32_t d) {
       V r = ar.mul(shfl64(bl, (int)(d & 63u)), shfl64(bh, (int)((d >> 6) & 63u)));
       if (S > 4096) r = ar.mul(r, shfl64(b3, (int)((d >> 12) & 63u)));
       if (S > 262144) r = ar.mul(r, shfl64(b4, (int)(d >> 18)));
@@ -1065,7 +1082,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     for (int a = 0; a < 2; a++) {
       const bool va = na > a, vb = nb > a;
       const uint32_t da = MS(ra[a] + S - nmod), db = MS(rb[a] + S - nmod);
-      const V ta = ar.mul(h0, binv_pow(da)), tb = ar.mul(h1, binv_pow(db));
+      const V ta = ar.mul(h0, bdig_pow(da)), tb = ar.mul(h1, bdig_pow(db));
       const uint64_t m0 = __ballot(va), m1 = __ballot(vb);
       const uint32_t c0 = __popcll(m0), c1 = __popcll(m1);
       if (va) {
@@ -1106,15 +1123,15 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     cS = S;
     cE = (uint32_t)E;
     cM = (uint32_t)M;
-    // the giant step b^-(S*E) and the table stride b^S (kept in bT[1]) by
+    // the giant step b^(S*E) and the table stride b^-S (kept in bT[1]) by
     // lanes 0 and 1, then their T-th powers by log2(T) squarings: the giant
-    // step of a workgroup round b^-(S*E*T) and the baby stride b^(S*T)
+    // step of a workgroup round b^(S*E*T) and the baby stride b^-(S*T)
     uint64_t ye2 = 0;
-    V yb2 = binv;
+    V yb2 = mbase;
     if (lane == 0) ye2 = SE;
     else if (lane == 1) {
       ye2 = S;
-      yb2 = mbase;
+      yb2 = binv;
     }
     V y2 = vone;
     if (lane < 2) y2 = ar.pow_u64(yb2, ye2);
