@@ -242,7 +242,10 @@ class SplitModel(CostModel):
         of = np.log2(np.array(self.options, np.float32))
         return np.concatenate([np.repeat(X, k, 0), np.tile(of, (len(X), 1))], 1).astype(np.float32)
 
-    def fit(self, X, Y, V, epochs=800, batch=256, lr=2e-3, seed=1):
+    # 2000 epochs: the lowest 4-fold cross-validated regret on the 256
+    # training jobs among widths 8, 16, 32 and 800-4000 epochs (0.20 %, 0.68 %
+    # at 800 epochs)
+    def fit(self, X, Y, V, epochs=2000, batch=256, lr=2e-3, seed=1):
         A = self._expand(X)
         self.mu = A.mean(0).astype(np.float32)
         self.sd = (A.std(0) + 1e-6).astype(np.float32)
