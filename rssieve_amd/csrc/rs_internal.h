@@ -24,7 +24,7 @@ struct JobDev {
   uint32_t threads;           // BSGS workgroup size
   uint32_t record_factors;    // 0: claims clear bitmap bits only
   uint64_t factor_cap;
-  uint32_t qcap;              // filter positives queued per BSGS workgroup (1..384)
+  uint32_t qcap;              // filter positives queued per BSGS workgroup (1..RS_QUEUE_CAP)
   uint32_t pad;
 };
 
@@ -99,6 +99,7 @@ struct SegJob {
 };
 
 #define RS_MAX_BATCH 256
+#define RS_QUEUE_CAP 128  // filter positives a BSGS workgroup can queue in LDS
 
 // Work queue of one round over all jobs of a batch.
 struct BatchState {

@@ -24,6 +24,7 @@
 //
 // Because the cursors live on the device, the round can be captured once in
 // a HIP graph and replayed until every job's range is done.
+#include <atomic>
 #include <cstddef>
 #include <mutex>
 #include <type_traits>
@@ -1403,16 +1404,28 @@ __device__ __noinline__ int gtab_find_at(const uint32_t* tab, uint32_t bmask, ui
 // flight).  s = 17 for p >= 2^45; below, the fields move down so that they
 // stay inside the residue's bits, but never into the word index (for small p
 // they then cover fewer residue bits, which only costs selectivity).
+//
+// The filter starts at the fixed LDS byte address kFltBase, above the
+// kernel's static LDS (the host checks that the static LDS fits below it and
+// sizes the dynamic LDS so that the filter ends inside the workgroup's
+// allocation).  A constant base goes into the LDS instructions' offset field,
+// so a word's address is the one AND; the dynamic LDS would start at the
+// static size, an address the compiler does not fold (one more add per
+// baby step and per probe).
+constexpr uint32_t kFltBase = 8192;
+using lds_u32 = __attribute__((address_space(3))) uint32_t;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+using lds_u4 = __attribute__((address_space(3))) u32x4;
+using lds_u8 = __attribute__((address_space(3))) uint8_t;
+__device__ __forceinline__ lds_u32* flt_base() { return (lds_u32*)(uintptr_t)kFltBase; }
 struct Filter {
-  uint32_t* w;     // hsize words
   uint32_t bmask;  // (hsize - 1) << 2
   uint32_t s;      // first bit of the four 3-bit fields
 };
-__device__ __forceinline__ Filter filter_make(uint32_t* w, uint32_t hsize, uint64_t p) {
+__device__ __forceinline__ Filter filter_make(uint32_t hsize, uint64_t p) {
   const uint32_t lgh = 31u - (uint32_t)__clz((int)hsize);
   const uint32_t plen = 64u - (uint32_t)__clzll((long long)p);
   Filter f;
-  f.w = w;
   f.bmask = (hsize - 1u) << 2;
   f.s = max(2u + lgh, min(17u, plen > 28u ? plen - 28u : 0u));
   return f;
@@ -1421,10 +1434,12 @@ __device__ __forceinline__ uint32_t flt_bits(const Filter& f, uint64_t v) {
   const uint32_t sel = __builtin_amdgcn_alignbit((uint32_t)(v >> 32), (uint32_t)v, f.s) & 0x07070707u;
   return __builtin_amdgcn_perm(0x80402010u, 0x08040201u, sel);
 }
-__device__ __forceinline__ uint32_t* flt_word(const Filter& f, uint64_t v) {
-  return (uint32_t*)((char*)f.w + ((uint32_t)v & f.bmask));
+__device__ __forceinline__ lds_u32* flt_word(const Filter& f, uint64_t v) {
+  return (lds_u32*)((lds_u8*)flt_base() + ((uint32_t)v & f.bmask));
 }
-__device__ __forceinline__ void flt_insert(const Filter& f, uint64_t v) { atomicOr(flt_word(f, v), flt_bits(f, v)); }
+__device__ __forceinline__ void flt_insert(const Filter& f, uint64_t v) {
+  __hip_atomic_fetch_or(flt_word(f, v), flt_bits(f, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 // The giant-step products of the tile loop in two arithmetics: binary64 plain
 // residues for odd p < 2^51 (a probe's key is kFpBias + the residue, the
@@ -1459,7 +1474,7 @@ struct GiantInt {
 // reuse the main filter's fields: a false positive passed the main filter
 // because some baby value shares its word and bit fields, and that baby value
 // would then pass here too.  A multiplicative mix of all 64 bits, read from
-// its upper bits (which depend on every input bit): word index bits 22..31,
+// its upper bits (which depend on every input bit): word index bits 23..31,
 // bit positions from bits 7..21.
 __device__ __forceinline__ uint32_t qmix(uint64_t v) {
   return ((uint32_t)v ^ ((uint32_t)(v >> 32) * 0x85EBCA77u)) * 0x9E3779B1u;
@@ -1502,10 +1517,13 @@ __device__ __forceinline__ uint32_t guide_start(const Guide& g, uint32_t k) {
 // at the end of the item), instead of one lane at a time inside the loop.
 // 384 entries keep the static LDS at 13 KB, so a 64 KB filter still fits two
 // 256-thread workgroups per CU.
-constexpr uint32_t kQCap = 384;
-constexpr uint32_t kHQ = 1024;  // filter words over the queue (qmix gives 10 index bits)
+constexpr uint32_t kQCap = RS_QUEUE_CAP;
+constexpr uint32_t kHQ = 512;  // filter words over the queue (qmix gives 9 index bits)
 constexpr uint32_t kHitCap = 64;
-constexpr uint32_t kWCap = 257;  // baby values of entries 64j: j <= ceil(16383 / 64)
+// the discrete-log hash holds j + 1 <= ceil(16383 / 64) + 1 in 9 bits and at
+// most 257 entries in its kHQ slots (fewer than kHQ: some bucket keeps an
+// empty last slot, which ends every search)
+static_assert(16383 / 64 + 2 < 512 && 257 < kHQ, "discrete-log hash entries");
 
 // One persistent kernel serves every job of a batch.  An item is (job, prime,
 // class) for a job with the reference's three tables (giant.cl giantfull /
@@ -1517,17 +1535,19 @@ constexpr uint32_t kWCap = 257;  // baby values of entries 64j: j <= ceil(16383 
 // CU (three waves per SIMD, at most 168 VGPRs), two 512-thread or one
 // 1024-thread workgroup (four waves per SIMD, at most 128).
 template <int T>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 : 3))) void k_bsgs(
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 : 3))) void k_bsgs(
     SegJob* __restrict__ jobs, BatchState* __restrict__ bs) {
-  // targets per giant-step tile: six at T <= 256 keep the kernel within 168
-  // VGPRs (eight spilled and ran 6 % slower), four at T >= 512 (128 VGPRs)
-  constexpr uint32_t kU = T <= 256 ? 6 : 4;
+  // Four wavefronts per SIMD from 256 threads on (128 VGPRs, a few spilled):
+  // at 256 threads four 40 KiB workgroups then share a CU's LDS with filters
+  // of 8192 words, which hides more of the items' latencies than three
+  // workgroups at 139 VGPRs without spills (44.5 against 47.2 ms per launch
+  // at the production shape).  Targets per giant-step tile: four (43.3 ms;
+  // five the same, three 44.0, six 44.5), six at 128 threads.
+  constexpr uint32_t kU = T <= 128 ? 6 : 4;
   constexpr uint32_t kW = T / 64;  // wavefronts per workgroup
   constexpr int kLgT = T >= 1024 ? 10 : T >= 512 ? 9 : T >= 256 ? 8 : 7;  // log2(T)
   static_assert((1 << kLgT) == T, "T is a power of two from 128 to 1024");
   __shared__ uint64_t s_pw[2][kLgT];  // per-item power tables (see vstart)
-  extern __shared__ uint4 lds4[];
-  uint32_t* tab = (uint32_t*)lds4;  // the filter; the bucketed table of the overflow recovery reuses it
   __shared__ uint64_t s_h[2 * RS_MAXK + 24];  // targets of the item (+ the last tile's pad, rg*kU <= 24)
   static_assert(RS_MAXK <= 256, "a sequence index fits the low 8 bits of s_k");
   __shared__ uint32_t s_k[2 * RS_MAXK];   // sequence index | shift (< 2^21) << 8
@@ -1557,9 +1577,6 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
   static_assert(kHitCap * sizeof(uint2) == 64 * sizeof(uint64_t), "s_gl and s_hit share one array");
   __shared__ uint64_t s_gl[64];
   uint2* const s_hit = (uint2*)s_gl;
-  // baby values of the entries 64j (the positives' discrete logs): binary64
-  // keys (kFpBias + residue) on the binary64 path, Montgomery residues otherwise
-  __shared__ uint64_t s_W[kWCap];
   __shared__ uint64_t s_pf[64];     // b^(stride*f), f < 64 (the positives' discrete logs)
   __shared__ uint32_t s_nhit;
   const uint32_t tid = threadIdx.x;
@@ -1672,7 +1689,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     const int K = (int)rfl(s_jc.kcount);
     const uint32_t nmin = rfl(s_jc.nmin), nmax = rfl(s_jc.nmax);
     const uint32_t cp = (c && !merged) ? 1 : 0;
-    const Filter flt = filter_make(tab, hsize, p);
+    const Filter flt = filter_make(hsize, p);
     // table stride, baby-step entries, exponent span and giant steps of the
     // item: the reference's (three lists), fixed (merged), or per prime
     // (residue classes)
@@ -1692,7 +1709,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     uint32_t* __restrict__ bitmap = rflp(s_jc.bitmap);
     FactorRec* __restrict__ fac = rflp(s_jc.fac);
     Counters* __restrict__ cnt = rflp(s_jc.cnt);
-    const uint32_t qcap = rfl(s_jc.qcap);
+    const uint32_t qcap = min(rfl(s_jc.qcap), kQCap);  // (the host never passes more)
     // Queue a filter positive.  Past qcap the entry is dropped and the
     // item's remaining rounds are redone exactly (settle); the hot loop
     // itself holds no table lookups, calls or global accesses.
@@ -1807,14 +1824,14 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       for (uint32_t s = tid; s < nq; s += T) {
         const uint4 qe = s_q[s];
         const uint32_t h = qmix(((uint64_t)qe.y << 32) | qe.x);
-        atomicOr(&s_hq[h >> 22], qmix_bits(h));
+        atomicOr(&s_hq[h >> 23], qmix_bits(h));
       }
       __syncthreads();
       // queued values equal to baby value v: slots s with fn(e, s)
       auto scan = [&](uint32_t e, uint64_t v, auto&& fn) {
         const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
         const uint32_t h = qmix(v), m = qmix_bits(h);
-        if (!lo || (s_hq[h >> 22] & m) != m) return;
+        if (!lo || (s_hq[h >> 23] & m) != m) return;
         for (uint32_t s = 0; s < nq; s++) {
           const uint4 qe = s_q[s];
           if (qe.x == lo && qe.y == hi) fn(e, s);
@@ -1843,14 +1860,19 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     // shortcut, one discrete logarithm each.  With f = lane (0..63), a queued
     // value g is the baby value of entry e = 64j - f exactly when
     // g * b^(stride*f) = W_j, the baby value of entry 64j, for the one j in
-    // [0, J], J = ceil((E - 1) / 64).  The baby walk leaves W_j (64j < E) in
-    // s_W, and one product gives W_J when 64J >= E (whash_build, after the
-    // walk; complete at the first settle's barrier).  The hash s_hq holds a
-    // 23-bit fingerprint of each W_j and j + 1 in buckets of four slots
-    // (kHB buckets picked by residue bits 2..9, linear probing over buckets;
-    // a bucket fills its slots in order, so a search ends at the first bucket
-    // with an empty last slot), and a fingerprint hit is compared with s_W[j]
-    // in full, so a hit is exact.  Every wavefront takes every kW-th queued
+    // [0, J], J = ceil((E - 1) / 64).  The hash s_hq holds a 23-bit
+    // fingerprint of each W_j and j + 1 in buckets of four slots (kHB buckets
+    // picked by residue bits 2..8, linear probing over buckets; a bucket
+    // fills its slots in order, so a search ends at the first bucket with an
+    // empty last slot).  Lane 0 of the baby walk writes each W_j's bucket
+    // and fingerprint (32 bits; 64j < E) as it passes entry 64j, and W_J's
+    // from W_(J-1) * b^(64*stride) when 64J >= E, to s_wk, which borrows the
+    // positive queue's LDS (idle until the giant steps); after the walk every
+    // thread inserts at most two of them (whash_build, one returning LDS
+    // atomic each, then a barrier before the queue is used).  A fingerprint
+    // hit, rare unless it is a solution, is compared in full with W_j = v0 *
+    // (b^(64*stride))^j, computed then, so a hit is exact and the values
+    // need no LDS.  Every wavefront takes every kW-th queued
     // value, two at a time: per lane and value one product (by b^(stride*f)
     // from s_pf) and, nearly always, one 16-byte LDS read.  A value whose low
     // 32 bits are 0 is never a hit (the reference's table cannot hold it,
@@ -1858,24 +1880,34 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     constexpr uint32_t kHB = kHQ / 4;
     const uint32_t nJ = nent > 1 ? (nent - 1 + 63) / 64 : 0;  // J
     auto wfp = [](uint64_t w) { return (uint32_t)(w >> 20) & 0x7FFFFFu; };
+    static_assert(kHB == 128 && kQCap * 4 >= 258, "s_wk: a 7-bit bucket and 258 entries in the queue's LDS");
+    uint32_t* const s_wk = (uint32_t*)s_q;
+    auto winfo = [&](uint64_t w) { return (wfp(w) << 7) | (((uint32_t)w >> 2) & (kHB - 1)); };
+    // W_j from the walk (lane 0, entry 64j), and W_J after W_(J-1) when 64J >= E
+    auto w_put = [&](uint32_t j, uint64_t w) {
+      s_wk[j] = winfo(w);
+      if (j + 1 == nJ && 64 * nJ >= nent)
+        s_wk[nJ] = winfo(fp ? fp_mulmod_bits(__longlong_as_double((long long)w) - kFpTwo52,
+                                             __longlong_as_double(s_pw[0][6]), pd, pinv, pd + kFpTwo52)
+                            : mmul(w, s_pw[0][6], p, q));
+    };
     auto whash_build = [&]() {
       for (uint32_t j = tid; j <= nJ; j += T) {
-        uint64_t w;
-        if (nJ && j == nJ && 64 * nJ >= nent) {  // W_J past the table: W_(J-1) * b^(64*stride)
-          w = fp ? fp_mulmod_bits(__longlong_as_double((long long)s_W[nJ - 1]) - kFpTwo52,
-                                  __longlong_as_double(s_pw[0][6]), pd, pinv, pd + kFpTwo52)
-                 : mmul(s_W[nJ - 1], s_pw[0][6], p, q);
-          s_W[nJ] = w;
-        } else {
-          w = s_W[j];
-        }
-        const uint32_t entry = (wfp(w) << 9) | (j + 1);
-        for (uint32_t bk = ((uint32_t)w >> 2) & (kHB - 1);; bk = (bk + 1) & (kHB - 1)) {
+        const uint32_t wi = s_wk[j], entry = ((wi >> 7) << 9) | (j + 1);
+        for (uint32_t bk = wi & (kHB - 1);; bk = (bk + 1) & (kHB - 1)) {
           uint32_t k = 0;
           while (k < 4 && atomicCAS(&s_hq[4 * bk + k], 0u, entry) != 0u) k++;
           if (k < 4) break;
         }
       }
+    };
+    // W_j in the form the walk gives it (binary64 keys kFpBias + residue, or
+    // Montgomery residues)
+    auto w_value = [&](uint32_t j) -> uint64_t {
+      if (fp)
+        return fp_mulmod_bits((double)v0, fp_powmod_sm(__longlong_as_double(s_pw[0][6]), j, pd, pinv), pd, pinv,
+                              pd + kFpTwo52);
+      return mmul(v0, powmod_sm(s_pw[0][6], j, p, q, one), p, q);
     };
     auto dlog = [&](uint32_t nq) {
       const uint64_t plane = s_pf[lane];
@@ -1887,7 +1919,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
           if (sl[k] && (sl[k] >> 9) == fg) {
             const uint32_t j = (sl[k] & 511u) - 1;
             const uint32_t e = 64 * j - lane;
-            if (64 * j >= lane && e < nent && (uint32_t)g && s_W[j] == gam) claim_hit(e, sq, 0u);
+            if (64 * j >= lane && e < nent && (uint32_t)g && w_value(j) == gam) claim_hit(e, sq, 0u);
           }
         }
         return en.w != 0u;
@@ -1899,7 +1931,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
           en = ((const uint4*)s_hq)[bk];
         }
       };
-      auto lpow = [&](uint64_t g) {  // in the form s_W holds
+      auto lpow = [&](uint64_t g) {  // in the form of W_j
         return fp ? fp_mulmod_bits((double)g, __longlong_as_double(plane), pd, pinv, pd + kFpTwo52)
                   : mmul(g, plane, p, q);
       };
@@ -1976,7 +2008,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
 
     {
       const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-      for (uint32_t s = tid; s < hsize / 4; s += T) lds4[s] = z;  // the filter
+      for (uint32_t s = tid; s < hsize / 4; s += T) ((lds_u4*)flt_base())[s] = u32x4{0u, 0u, 0u, 0u};  // the filter
       for (uint32_t s = tid; s < kHQ / 4; s += T) ((uint4*)s_hq)[s] = z;  // the discrete-log hash
     }
     // targets: Montgomery residues, as binary64 values on the binary64 path
@@ -1991,8 +2023,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
     __syncthreads();
     // vstart = v0 * b^(stride*lane) * b^(stride*64*wave): the lane's factor
     // as a product tree over its six bits (wavefront 0 keeps it in s_pf for
-    // the discrete logs), the wavefront's bits by uniform branches; wavefront
-    // 0 also writes G^lane to s_gl (the giant steps' first tiles)
+    // the discrete logs), the wavefront's bits by uniform branches; the last
+    // wavefront (not wavefront 0, which computes the power tables and runs
+    // the item fetch) also writes G^lane to s_gl (the giant steps' first tiles)
     if (fp) {
       double f[6];
 #pragma unroll
@@ -2001,7 +2034,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
                    f45 = fp_mulmod_signed(f[4], f[5], pd, pinv);
       const double fl = fp_mulmod_signed(fp_mulmod_signed(f01, f23, pd, pinv), f45, pd, pinv);
       if (tid < 64) s_pf[tid] = (uint64_t)__double_as_longlong(fl);
-      if (wave == 0) {
+      if (wave == kW - 1) {
         double g[6];
 #pragma unroll
         for (int i = 0; i < 6; i++) g[i] = ((lane >> i) & 1u) ? __longlong_as_double(s_pw[1][i]) : 1.0;
@@ -2021,7 +2054,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
       for (int i = 0; i < 6; i++) f[i] = ((lane >> i) & 1u) ? s_pw[0][i] : one;
       const uint64_t fl = mmul(mmul(mmul(f[0], f[1], p, q), mmul(f[2], f[3], p, q), p, q), mmul(f[4], f[5], p, q), p, q);
       if (tid < 64) s_pf[tid] = fl;
-      if (wave == 0) {
+      if (wave == kW - 1) {
         uint64_t g[6];
 #pragma unroll
         for (int i = 0; i < 6; i++) g[i] = ((lane >> i) & 1u) ? s_pw[1][i] : one;
@@ -2050,7 +2083,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         pf = ld_global(a);
       }
     }
-    // Baby steps: set the filter bits, keep W_j, the value of entry 64j
+    // Baby steps: set the filter bits, hash W_j, the value of entry 64j
     // (lane 0's values: entry e = 64*wave + T*i is W_(wave + kW*i)), and find
     // ord_p(b) below the span (a repeat of v0).  A value whose low 32 bits
     // are 0 is inserted like any other: its probes pass the filter and are
@@ -2080,7 +2113,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         }
         if (lane == 0) {
 #pragma unroll
-          for (int c = 0; c < 4; c++) s_W[jw + c * kW] = key[c];
+          for (int c = 0; c < 4; c++) w_put(jw + c * kW, key[c]);
         }
 #pragma unroll
         for (int c = 0; c < 4; c++) v[c] = fp_mulmod_signed(v[c], b4, pd, pinv);
@@ -2090,14 +2123,14 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         if (e + c * T < nent) {
           const uint64_t key = fp_signed_bits(v[c], pc);
           visit(e + c * T, key);
-          if (lane == 0) s_W[jw + c * kW] = key;
+          if (lane == 0) w_put(jw + c * kW, key);
         }
       }
     } else {
       baby_walk(nent, [&](uint32_t e, uint64_t v) {
         if (v == v0) atomicMin(&s_order, e * stride - 1u);
         flt_insert(flt, v);
-        if (lane == 0) s_W[e >> 6] = v;  // e = 64 * (e >> 6) on lane 0 (T is a multiple of 64)
+        if (lane == 0) w_put(e >> 6, v);  // e = 64 * (e >> 6) on lane 0 (T is a multiple of 64)
       });
     }
     __syncthreads();
@@ -2140,14 +2173,14 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         Tg tt[kU];
         auto load_t = [&]() {
 #pragma unroll
-          for (uint32_t k = 0; k < kU; k++) tt[k] = ar.target(s_h[(g * rg + lgrp) * kU + k]);
+          for (uint32_t k = 0; k < kU; k++) tt[k] = ar.target(s_h[__umul24(g * rg + lgrp, kU) + k]);
         };
         // products of the current tile (targets in tt), then advance to the
         // wavefront's next tile
         auto mul_t = [&](uint64_t(&uu)[kU], uint32_t& i0, uint32_t& qbt) {
 #pragma unroll
           for (uint32_t k = 0; k < kU; k++) uu[k] = ar.key(tt[k], gq);
-          i0 = (g * rg + lgrp) * kU;
+          i0 = __umul24(g * rg + lgrp, kU);  // (24-bit products: a full-rate multiply)
           qbt = qb;
           g = (uint32_t)__builtin_amdgcn_readfirstlane((int)(g + kW));
           while (g >= ng) {
@@ -2195,6 +2228,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 512 ? 4 
         }
       };
       whash_build();
+      __syncthreads();  // the hash complete, s_wk read: the queue's LDS is free
       if (fp) {
         GiantFp ar;
         ar.pd = pd;
@@ -2244,22 +2278,52 @@ __global__ void k_round_items(SegJob* __restrict__ jobs, BatchState* __restrict_
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-// dynamic LDS of a BSGS workgroup: the filter (hsize u32)
-size_t bsgs_lds_bytes(uint32_t hsize, uint32_t) { return (size_t)hsize * 4; }
-
-// static LDS of the BSGS kernel (target lists, positive queue, counters)
+// static LDS of the BSGS kernel (target lists, positive queue, counters),
+// queried once per workgroup size
 hipError_t bsgs_static_lds(uint32_t threads, size_t* out) {
-  hipFuncAttributes a;
-  hipError_t e;
-  switch (threads) {
-    case 128: e = hipFuncGetAttributes(&a, (const void*)k_bsgs<128>); break;
-    case 256: e = hipFuncGetAttributes(&a, (const void*)k_bsgs<256>); break;
-    case 512: e = hipFuncGetAttributes(&a, (const void*)k_bsgs<512>); break;
-    case 1024: e = hipFuncGetAttributes(&a, (const void*)k_bsgs<1024>); break;
-    default: return hipErrorInvalidValue;
+  static std::atomic<size_t> cache[4];  // 0: not queried yet (the static LDS is never empty)
+  const int slot = threads == 128 ? 0 : threads == 256 ? 1 : threads == 512 ? 2 : threads == 1024 ? 3 : -1;
+  if (slot < 0) return hipErrorInvalidValue;
+  size_t v = cache[slot].load(std::memory_order_relaxed);
+  if (!v) {
+    hipFuncAttributes a;
+    hipError_t e;
+    switch (threads) {
+      case 128: e = hipFuncGetAttributes(&a, (const void*)k_bsgs<128>); break;
+      case 256: e = hipFuncGetAttributes(&a, (const void*)k_bsgs<256>); break;
+      case 512: e = hipFuncGetAttributes(&a, (const void*)k_bsgs<512>); break;
+      default: e = hipFuncGetAttributes(&a, (const void*)k_bsgs<1024>); break;
+    }
+    if (e != hipSuccess) return e;
+    v = a.sharedSizeBytes;
+    cache[slot].store(v, std::memory_order_relaxed);
   }
-  if (e == hipSuccess) *out = a.sharedSizeBytes;
-  return e;
+  *out = v;
+  return hipSuccess;
+}
+
+// LDS of one BSGS workgroup in all: the filter ends at kFltBase + 4*hsize
+size_t bsgs_block_lds_bytes(uint32_t hsize) { return (size_t)kFltBase + (size_t)hsize * 4; }
+
+// dynamic LDS of a BSGS workgroup: from the end of the static LDS to the end
+// of the filter (the padding below kFltBase included)
+size_t bsgs_lds_bytes(uint32_t hsize, uint32_t threads) {
+  size_t st = 0;
+  if (bsgs_static_lds(threads, &st) != hipSuccess || st > kFltBase) st = 0;  // refused by bsgs_check_static_lds
+  return bsgs_block_lds_bytes(hsize) - st;
+}
+
+// every BSGS kernel's static LDS must end below the filter's fixed base
+hipError_t bsgs_check_static_lds(size_t* worst) {
+  size_t w = 0;
+  for (uint32_t t : {128u, 256u, 512u, 1024u}) {
+    size_t st = 0;
+    const hipError_t e = bsgs_static_lds(t, &st);
+    if (e != hipSuccess) return e;
+    w = st > w ? st : w;
+  }
+  *worst = w;
+  return w <= kFltBase ? hipSuccess : hipErrorInvalidKernelFile;
 }
 
 hipError_t bsgs_blocks_per_cu(uint32_t threads, uint32_t hsize, int* out) {

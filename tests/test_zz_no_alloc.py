@@ -47,7 +47,7 @@ FIRST = (2, 10, 100000, 199999, 10**9, 10**9 + 3 * 10**6, 41, R.ALGO_THREE_LISTS
 SECOND = (3, 4, 20000, 50000, 10**7, 10**7 + 5 * 10**5, 42, R.ALGO_CLASSES, 2048, 4096, 256)
 
 
-@pytest.mark.parametrize("qcap", [1, 0], ids=["queue1", "queue384"])
+@pytest.mark.parametrize("qcap", [1, 0], ids=["queue1", "queue_max"])
 def test_noalloc_matches_rs_sieve_on_two_calls(ctx, qcap):
     for case in (FIRST, SECOND):
         base, K, nmin, nmax, pmin, pmax, seed, algo, Q, hsize, threads = case
