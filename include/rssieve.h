@@ -85,7 +85,8 @@ typedef struct {
   int32_t record_factors;  /* 0: claims only clear bitmap bits (pre-sieving) */
   uint64_t factor_cap;     /* factor records held on the device between drains */
   uint32_t queue_cap;      /* filter positives a BSGS workgroup queues before resolving them (0: the
-                              maximum, 128); a queue that overflows is recovered exactly by redoing its
+                              maximum: 128 entries at 128 and 256 threads, 384 at 512 and 1024; more is
+                              refused); a queue that overflows is recovered exactly by redoing its
                               giant steps, so small values only cost time (they exercise that path) */
   uint32_t reserved;       /* 0 */
 } rs_params;

@@ -18,11 +18,11 @@
 
 namespace rs {
 hipError_t bsgs_blocks_per_cu(uint32_t threads, uint32_t hsize, int* out);
-size_t bsgs_block_lds_bytes(uint32_t hsize);
+size_t bsgs_block_lds_bytes(uint32_t hsize, uint32_t threads);
 hipError_t bsgs_check_static_lds(size_t* worst);
 // dynamic + static LDS of one BSGS workgroup (the filter ends at a fixed
 // address above the static LDS, checked at rs_open)
-static size_t bsgs_block_lds(uint32_t hsize, uint32_t) { return bsgs_block_lds_bytes(hsize); }
+static size_t bsgs_block_lds(uint32_t hsize, uint32_t threads) { return bsgs_block_lds_bytes(hsize, threads); }
 hipError_t launch_set_cursor(SegJob* sj, uint64_t pmin, uint64_t pmax, uint32_t threads, uint64_t range,
                              uint32_t gen_blocks, hipStream_t s);
 hipError_t launch_round(SegJob* jobs, BatchState* bs, uint32_t njobs, uint32_t gen_blocks, uint32_t setup_blocks,
@@ -150,7 +150,7 @@ extern "C" int rs_open(int device, rs_ctx** out) {
   if (e != hipSuccess) {
     (void)hipStreamDestroy(c->stream);
     delete c;
-    return fail(RS_E_HIP, "rs_open: BSGS kernels' static LDS (%zu bytes) overlaps the filter base: %s", st_lds,
+    return fail(RS_E_HIP, "rs_open: a BSGS kernel's static LDS reaches %zu bytes into its filter: %s", st_lds,
                 hipGetErrorString(e));
   }
   e = static_scratch_address(device, &c->static_scratch);
@@ -280,8 +280,9 @@ static int check_params(const rs_ctx* ctx, const rs_params* prm, const char* who
   if (prm->Q < 16 || entries > 16384 || !pow2(prm->hsize) || prm->hsize < 64 || prm->hsize < entries)
     return fail(RS_E_ARG, "%s: need at most 16384 baby-step entries and a power-of-two filter of at least 64 "
                 "words and at least one word per entry (Q=%u entries=%u hsize=%u)", who, prm->Q, entries, prm->hsize);
-  if (prm->queue_cap > RS_QUEUE_CAP)
-    return fail(RS_E_ARG, "%s: queue_cap %u above %u", who, prm->queue_cap, (unsigned)RS_QUEUE_CAP);
+  if (prm->queue_cap > bsgs_queue_cap(prm->threads))
+    return fail(RS_E_ARG, "%s: queue_cap %u above the %u entries of a %u-thread workgroup", who, prm->queue_cap,
+                bsgs_queue_cap(prm->threads), prm->threads);
   if (bsgs_block_lds(prm->hsize, prm->threads) > ctx->lds_per_block)
     return fail(RS_E_ARG, "%s: hash of %u slots needs %zu LDS bytes, device has %u", who, prm->hsize,
                 bsgs_block_lds(prm->hsize, prm->threads), ctx->lds_per_block);
@@ -302,7 +303,7 @@ static int apply_params(rs_job* j, const rs_params* prm) {
   h.hsize = prm->hsize;
   h.hmask = prm->hsize - 1;
   h.threads = prm->threads;
-  h.qcap = prm->queue_cap ? prm->queue_cap : RS_QUEUE_CAP;
+  h.qcap = prm->queue_cap;  // 0: the kernel's queue size
   int bpc = 0;
   if (bsgs_blocks_per_cu(prm->threads, prm->hsize, &bpc) != hipSuccess || bpc < 1) bpc = 1;
   if (prm->blocks_per_cu && (int)prm->blocks_per_cu < bpc) bpc = (int)prm->blocks_per_cu;

@@ -24,7 +24,7 @@ struct JobDev {
   uint32_t threads;           // BSGS workgroup size
   uint32_t record_factors;    // 0: claims clear bitmap bits only
   uint64_t factor_cap;
-  uint32_t qcap;              // filter positives queued per BSGS workgroup (1..RS_QUEUE_CAP)
+  uint32_t qcap;              // filter positives queued per BSGS workgroup (0: the kernel's queue size)
   uint32_t pad;
 };
 
@@ -99,7 +99,25 @@ struct SegJob {
 };
 
 #define RS_MAX_BATCH 256
-#define RS_QUEUE_CAP 128  // filter positives a BSGS workgroup can queue in LDS
+// LDS shape of the BSGS kernel by workgroup size: the positive queue
+// (entries), the queue filter / discrete-log hash (32-bit words) and the
+// fixed LDS byte address of the Bloom filter, above the static LDS.  At 128
+// and 256 threads the static LDS stays within 8 KiB, so four workgroups with
+// 32 KiB filters share a CU's 160 KiB; at 512 and 1024 threads the register
+// budget allows no more workgroups than a 16 KiB base does, and the larger
+// queue is checked less often.
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+constexpr uint32_t bsgs_queue_cap(uint32_t threads) { return threads >= 512 ? 384u : 128u; }
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+constexpr uint32_t bsgs_hash_words(uint32_t threads) { return threads >= 512 ? 1024u : 512u; }
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+constexpr uint32_t bsgs_filter_base(uint32_t threads) { return threads >= 512 ? 16384u : 8192u; }
 
 // Work queue of one round over all jobs of a batch.
 struct BatchState {

@@ -1405,19 +1405,17 @@ __device__ __noinline__ int gtab_find_at(const uint32_t* tab, uint32_t bmask, ui
 // stay inside the residue's bits, but never into the word index (for small p
 // they then cover fewer residue bits, which only costs selectivity).
 //
-// The filter starts at the fixed LDS byte address kFltBase, above the
-// kernel's static LDS (the host checks that the static LDS fits below it and
-// sizes the dynamic LDS so that the filter ends inside the workgroup's
+// The filter starts at a fixed LDS byte address, bsgs_filter_base(T), above
+// the kernel's static LDS (the host checks that the static LDS fits below it
+// and sizes the dynamic LDS so that the filter ends inside the workgroup's
 // allocation).  A constant base goes into the LDS instructions' offset field,
 // so a word's address is the one AND; the dynamic LDS would start at the
 // static size, an address the compiler does not fold (one more add per
 // baby step and per probe).
-constexpr uint32_t kFltBase = 8192;
 using lds_u32 = __attribute__((address_space(3))) uint32_t;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 using lds_u4 = __attribute__((address_space(3))) u32x4;
 using lds_u8 = __attribute__((address_space(3))) uint8_t;
-__device__ __forceinline__ lds_u32* flt_base() { return (lds_u32*)(uintptr_t)kFltBase; }
 struct Filter {
   uint32_t bmask;  // (hsize - 1) << 2
   uint32_t s;      // first bit of the four 3-bit fields
@@ -1434,11 +1432,13 @@ __device__ __forceinline__ uint32_t flt_bits(const Filter& f, uint64_t v) {
   const uint32_t sel = __builtin_amdgcn_alignbit((uint32_t)(v >> 32), (uint32_t)v, f.s) & 0x07070707u;
   return __builtin_amdgcn_perm(0x80402010u, 0x08040201u, sel);
 }
+template <uint32_t B>
 __device__ __forceinline__ lds_u32* flt_word(const Filter& f, uint64_t v) {
-  return (lds_u32*)((lds_u8*)flt_base() + ((uint32_t)v & f.bmask));
+  return (lds_u32*)((lds_u8*)(uintptr_t)B + ((uint32_t)v & f.bmask));
 }
+template <uint32_t B>
 __device__ __forceinline__ void flt_insert(const Filter& f, uint64_t v) {
-  __hip_atomic_fetch_or(flt_word(f, v), flt_bits(f, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __hip_atomic_fetch_or(flt_word<B>(f, v), flt_bits(f, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // The giant-step products of the tile loop in two arithmetics: binary64 plain
@@ -1514,16 +1514,9 @@ __device__ __forceinline__ uint32_t guide_start(const Guide& g, uint32_t k) {
 
 // Filter positives are queued in LDS during the giant steps and resolved by
 // all threads together at round boundaries (once the queue is half full, and
-// at the end of the item), instead of one lane at a time inside the loop.
-// 384 entries keep the static LDS at 13 KB, so a 64 KB filter still fits two
-// 256-thread workgroups per CU.
-constexpr uint32_t kQCap = RS_QUEUE_CAP;
-constexpr uint32_t kHQ = 512;  // filter words over the queue (qmix gives 9 index bits)
+// at the end of the item), instead of one lane at a time inside the loop
+// (bsgs_queue_cap entries: 128 at 128 and 256 threads, 384 above).
 constexpr uint32_t kHitCap = 64;
-// the discrete-log hash holds j + 1 <= ceil(16383 / 64) + 1 in 9 bits and at
-// most 257 entries in its kHQ slots (fewer than kHQ: some bucket keeps an
-// empty last slot, which ends every search)
-static_assert(16383 / 64 + 2 < 512 && 257 < kHQ, "discrete-log hash entries");
 
 // One persistent kernel serves every job of a batch.  An item is (job, prime,
 // class) for a job with the reference's three tables (giant.cl giantfull /
@@ -1531,9 +1524,9 @@ static_assert(16383 / 64 + 2 < 512 && 257 < kHQ, "discrete-log hash entries");
 // b^(NMIN + 2j), whose targets carry their shift d.  The LDS holds the largest
 // table of the batch; each item uses its own job's table size.
 // Register budgets for the occupancy the LDS allows at the production table
-// (8192 filter words, 49 KB per workgroup): three 256-thread workgroups per
-// CU (three waves per SIMD, at most 168 VGPRs), two 512-thread or one
-// 1024-thread workgroup (four waves per SIMD, at most 128).
+// (8192 filter words, 40 KiB per 256-thread workgroup): four 256-thread or
+// two 512-thread workgroups or one 1024-thread workgroup per CU, four waves
+// per SIMD, at most 128 VGPRs.
 template <int T>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 : 3))) void k_bsgs(
     SegJob* __restrict__ jobs, BatchState* __restrict__ bs) {
@@ -1544,6 +1537,14 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
   // at the production shape).  Targets per giant-step tile: four (43.3 ms;
   // five the same, three 44.0, six 44.5), six at 128 threads.
   constexpr uint32_t kU = T <= 128 ? 6 : 4;
+  constexpr uint32_t kQCap = bsgs_queue_cap(T);
+  constexpr uint32_t kHQ = bsgs_hash_words(T);
+  constexpr uint32_t kHQShift = kHQ == 1024 ? 22 : 23;  // qmix's word index bits
+  constexpr uint32_t kFB = bsgs_filter_base(T);
+  // the discrete-log hash holds j + 1 <= ceil(16383 / 64) + 1 in 9 bits and at
+  // most 257 entries in its kHQ slots (fewer than kHQ: some bucket keeps an
+  // empty last slot, which ends every search)
+  static_assert(16383 / 64 + 2 < 512 && 257 < kHQ && (1u << (32 - kHQShift)) == kHQ, "discrete-log hash");
   constexpr uint32_t kW = T / 64;  // wavefronts per workgroup
   constexpr int kLgT = T >= 1024 ? 10 : T >= 512 ? 9 : T >= 256 ? 8 : 7;  // log2(T)
   static_assert((1 << kLgT) == T, "T is a power of two from 128 to 1024");
@@ -1709,7 +1710,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
     uint32_t* __restrict__ bitmap = rflp(s_jc.bitmap);
     FactorRec* __restrict__ fac = rflp(s_jc.fac);
     Counters* __restrict__ cnt = rflp(s_jc.cnt);
-    const uint32_t qcap = min(rfl(s_jc.qcap), kQCap);  // (the host never passes more)
+    const uint32_t jq = rfl(s_jc.qcap), qcap = jq ? min(jq, kQCap) : kQCap;  // (the host never passes more)
     // Queue a filter positive.  Past qcap the entry is dropped and the
     // item's remaining rounds are redone exactly (settle); the hot loop
     // itself holds no table lookups, calls or global accesses.
@@ -1824,14 +1825,14 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
       for (uint32_t s = tid; s < nq; s += T) {
         const uint4 qe = s_q[s];
         const uint32_t h = qmix(((uint64_t)qe.y << 32) | qe.x);
-        atomicOr(&s_hq[h >> 23], qmix_bits(h));
+        atomicOr(&s_hq[h >> kHQShift], qmix_bits(h));
       }
       __syncthreads();
       // queued values equal to baby value v: slots s with fn(e, s)
       auto scan = [&](uint32_t e, uint64_t v, auto&& fn) {
         const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
         const uint32_t h = qmix(v), m = qmix_bits(h);
-        if (!lo || (s_hq[h >> 23] & m) != m) return;
+        if (!lo || (s_hq[h >> kHQShift] & m) != m) return;
         for (uint32_t s = 0; s < nq; s++) {
           const uint4 qe = s_q[s];
           if (qe.x == lo && qe.y == hi) fn(e, s);
@@ -1880,9 +1881,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
     constexpr uint32_t kHB = kHQ / 4;
     const uint32_t nJ = nent > 1 ? (nent - 1 + 63) / 64 : 0;  // J
     auto wfp = [](uint64_t w) { return (uint32_t)(w >> 20) & 0x7FFFFFu; };
-    static_assert(kHB == 128 && kQCap * 4 >= 258, "s_wk: a 7-bit bucket and 258 entries in the queue's LDS");
+    constexpr uint32_t kHBBits = kHB == 256 ? 8 : 7;
+    static_assert((1u << kHBBits) == kHB && kQCap * 4 >= 258, "s_wk: the bucket and 258 entries in the queue's LDS");
     uint32_t* const s_wk = (uint32_t*)s_q;
-    auto winfo = [&](uint64_t w) { return (wfp(w) << 7) | (((uint32_t)w >> 2) & (kHB - 1)); };
+    auto winfo = [&](uint64_t w) { return (wfp(w) << kHBBits) | (((uint32_t)w >> 2) & (kHB - 1)); };
     // W_j from the walk (lane 0, entry 64j), and W_J after W_(J-1) when 64J >= E
     auto w_put = [&](uint32_t j, uint64_t w) {
       s_wk[j] = winfo(w);
@@ -1893,7 +1895,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
     };
     auto whash_build = [&]() {
       for (uint32_t j = tid; j <= nJ; j += T) {
-        const uint32_t wi = s_wk[j], entry = ((wi >> 7) << 9) | (j + 1);
+        const uint32_t wi = s_wk[j], entry = ((wi >> kHBBits) << 9) | (j + 1);
         for (uint32_t bk = wi & (kHB - 1);; bk = (bk + 1) & (kHB - 1)) {
           uint32_t k = 0;
           while (k < 4 && atomicCAS(&s_hq[4 * bk + k], 0u, entry) != 0u) k++;
@@ -2008,7 +2010,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
 
     {
       const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-      for (uint32_t s = tid; s < hsize / 4; s += T) ((lds_u4*)flt_base())[s] = u32x4{0u, 0u, 0u, 0u};  // the filter
+      for (uint32_t s = tid; s < hsize / 4; s += T) ((lds_u4*)(uintptr_t)kFB)[s] = u32x4{0u, 0u, 0u, 0u};  // the filter
       for (uint32_t s = tid; s < kHQ / 4; s += T) ((uint4*)s_hq)[s] = z;  // the discrete-log hash
     }
     // targets: Montgomery residues, as binary64 values on the binary64 path
@@ -2096,7 +2098,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
       const double pc = pd + kFpTwo52;
       auto visit = [&](uint32_t e, uint64_t key) {
         if (key == k0) atomicMin(&s_order, e * stride - 1u);  // entry 0 gives ~0: no repeat
-        flt_insert(flt, key);
+        flt_insert<kFB>(flt, key);
       };
       double v[4];
       v[0] = (double)vstart;
@@ -2129,7 +2131,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
     } else {
       baby_walk(nent, [&](uint32_t e, uint64_t v) {
         if (v == v0) atomicMin(&s_order, e * stride - 1u);
-        flt_insert(flt, v);
+        flt_insert<kFB>(flt, v);
         if (lane == 0) w_put(e >> 6, v);  // e = 64 * (e >> 6) on lane 0 (T is a multiple of 64)
       });
     }
@@ -2191,7 +2193,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
         };
         auto read_w = [&](const uint64_t(&uu)[kU], uint32_t(&ww)[kU]) {
 #pragma unroll
-          for (uint32_t k = 0; k < kU; k++) ww[k] = *flt_word(flt, uu[k]);
+          for (uint32_t k = 0; k < kU; k++) ww[k] = *flt_word<kFB>(flt, uu[k]);
         };
         auto consume = [&](const uint64_t(&uu)[kU], const uint32_t(&ww)[kU], uint32_t i0, uint32_t qbt) {
           uint32_t x[kU], mn = 0xFFFFFFFFu;
@@ -2302,28 +2304,35 @@ hipError_t bsgs_static_lds(uint32_t threads, size_t* out) {
   return hipSuccess;
 }
 
-// LDS of one BSGS workgroup in all: the filter ends at kFltBase + 4*hsize
-size_t bsgs_block_lds_bytes(uint32_t hsize) { return (size_t)kFltBase + (size_t)hsize * 4; }
-
-// dynamic LDS of a BSGS workgroup: from the end of the static LDS to the end
-// of the filter (the padding below kFltBase included)
-size_t bsgs_lds_bytes(uint32_t hsize, uint32_t threads) {
-  size_t st = 0;
-  if (bsgs_static_lds(threads, &st) != hipSuccess || st > kFltBase) st = 0;  // refused by bsgs_check_static_lds
-  return bsgs_block_lds_bytes(hsize) - st;
+// LDS of one BSGS workgroup in all: the filter ends at its base + 4*hsize
+size_t bsgs_block_lds_bytes(uint32_t hsize, uint32_t threads) {
+  return (size_t)bsgs_filter_base(threads) + (size_t)hsize * 4;
 }
 
-// every BSGS kernel's static LDS must end below the filter's fixed base
+// dynamic LDS of a BSGS workgroup: from the end of the static LDS to the end
+// of the filter (the padding below the filter's base included)
+size_t bsgs_lds_bytes(uint32_t hsize, uint32_t threads) {
+  size_t st = 0;
+  if (bsgs_static_lds(threads, &st) != hipSuccess || st > bsgs_filter_base(threads)) st = 0;  // refused at rs_open
+  return bsgs_block_lds_bytes(hsize, threads) - st;
+}
+
+// every BSGS kernel's static LDS must end below its filter's fixed base; the
+// largest excess (or 0) in *worst
 hipError_t bsgs_check_static_lds(size_t* worst) {
   size_t w = 0;
+  hipError_t r = hipSuccess;
   for (uint32_t t : {128u, 256u, 512u, 1024u}) {
     size_t st = 0;
     const hipError_t e = bsgs_static_lds(t, &st);
     if (e != hipSuccess) return e;
-    w = st > w ? st : w;
+    if (st > bsgs_filter_base(t)) {
+      r = hipErrorInvalidKernelFile;
+      w = st - bsgs_filter_base(t) > w ? st - bsgs_filter_base(t) : w;
+    }
   }
   *worst = w;
-  return w <= kFltBase ? hipSuccess : hipErrorInvalidKernelFile;
+  return r;
 }
 
 hipError_t bsgs_blocks_per_cu(uint32_t threads, uint32_t hsize, int* out) {
