@@ -346,7 +346,8 @@ __constant__ uint32_t g_rd[kNRd] = {
     1019, 1021};
 constexpr uint32_t kZV = 464;                 // za^i at [0, 81), zb^i at [128, 192), zc^i at [192, 336),
                                               // zd^i at [336, 464)
-constexpr uint32_t kZH = 896;                 // their hash slots: [0, 256), [256, 384), [384, 640), [640, 896)
+constexpr uint32_t kZH = 896;                 // their 16-bit hash slots: [0, 256), [256, 384), [384, 640),
+                                              // [640, 896), two per 32-bit word
 constexpr uint64_t kClassMinP = 1u << 20;     // smaller p keep the merged table's parity split; above it
                                               // the exponent (p-1)/L is at least 2 (L <= 720720)
 
@@ -909,7 +910,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     // the tables of za^i (i <= La, at zv[0..]), zb^i (i <= Lb, at zv[128..])
     // and zc^i (i <= Lc, at zv[192..]), and their orders sa, sb, sc (0: no
     // i > 0 with value 1 in range, possible only for a composite p)
-    for (uint32_t h = lane; h < kZH; h += 64) zh[h] = 0u;
+    for (uint32_t h = lane; h < kZH / 2; h += 64) zh[h] = 0u;
     const V za = ar.load(pre.v[kPvZa]), zb = ar.load(pre.v[kPvZb]), zc = ar.load(pre.v[kPvZc]);
     const V zd = ar.load(pre.v[kPvZd]);
     // z^i for i <= Lz at zv[vb + i], 64 per pass; the order of z in range
@@ -960,23 +961,25 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
                          ((80u % sa == 0 && 63u % sb == 0 && 143u % sc == 0 && (sd == 1 || sd == Ld)) ||
                           (gcd(sa, sb) == 1 && gcd(sab, sc) == 1 && gcd(sabc, sd) == 1));
     const uint32_t s = coprime ? sabc * sd : 0u;
+    // 16-bit slots, two per 32-bit word of zh (so that seven workgroups'
+    // tables share a CU's LDS): a slot is taken by a compare-and-swap of its
+    // word, retried while its own half stays empty
+    auto zh_put = [&](uint32_t hb, uint32_t h, uint32_t hm, uint32_t val) {
+      for (;; h = (h + 1) & hm) {
+        const uint32_t sl = hb + h, sh = (sl & 1u) * 16u;
+        uint32_t old = zh[sl >> 1];
+        while (((old >> sh) & 0xFFFFu) == 0u) {
+          const uint32_t got = atomicCAS(&zh[sl >> 1], old, old | (val << sh));
+          if (got == old) return;
+          old = got;
+        }
+      }
+    };
     if (s) {
-      for (uint32_t idx = lane; idx < sa; idx += 64) {
-        uint32_t h = (uint32_t)zv[idx] & 255u;
-        while (atomicCAS(&zh[h], 0u, idx + 1) != 0u) h = (h + 1) & 255u;
-      }
-      if (lane < sb) {
-        uint32_t h = (uint32_t)zv[128 + lane] & 127u;
-        while (atomicCAS(&zh[256 + h], 0u, lane + 1) != 0u) h = (h + 1) & 127u;
-      }
-      for (uint32_t idx = lane; Lc > 1 && idx < sc; idx += 64) {
-        uint32_t h = (uint32_t)zv[192 + idx] & 255u;
-        while (atomicCAS(&zh[384 + h], 0u, idx + 1) != 0u) h = (h + 1) & 255u;
-      }
-      for (uint32_t idx = lane; Ld > 1 && idx < Bd; idx += 64) {
-        uint32_t h = (uint32_t)zv[336 + idx] & 255u;
-        while (atomicCAS(&zh[640 + h], 0u, idx + 1) != 0u) h = (h + 1) & 255u;
-      }
+      for (uint32_t idx = lane; idx < sa; idx += 64) zh_put(0u, (uint32_t)zv[idx] & 255u, 255u, idx + 1);
+      if (lane < sb) zh_put(256u, (uint32_t)zv[128 + lane] & 127u, 127u, lane + 1);
+      for (uint32_t idx = lane; Lc > 1 && idx < sc; idx += 64) zh_put(384u, (uint32_t)zv[192 + idx] & 255u, 255u, idx + 1);
+      for (uint32_t idx = lane; Ld > 1 && idx < Bd; idx += 64) zh_put(640u, (uint32_t)zv[336 + idx] & 255u, 255u, idx + 1);
     }
     wave_sync();
     // sa^-1 mod sb, (sa*sb)^-1 mod sc and (sa*sb*sc)^-1 mod sd for the CRT
@@ -986,7 +989,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     // index of a key in one table (hash base hb, mask hm, values at vb), or -1
     auto lookup = [&](uint64_t key, uint32_t hb, uint32_t hm, uint32_t vb) -> int {
       for (uint32_t h = (uint32_t)key & hm;; h = (h + 1) & hm) {
-        const uint32_t sl = zh[hb + h];
+        const uint32_t sl = (zh[(hb + h) >> 1] >> (((hb + h) & 1u) * 16u)) & 0xFFFFu;
         if (!sl) return -1;
         if (zv[vb + sl - 1] == key) return (int)sl - 1;
       }
@@ -1250,14 +1253,15 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
   if (lane == 0) *R = rec;
 }
 
-// five wavefronts per SIMD (at most 96 VGPRs): the setup is issue-bound and
-// needs the waves to cover its FP64 and LDS latencies
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_setup(SegJob* __restrict__ jobs) {
+// seven wavefronts per SIMD (at most 73 VGPRs; 21.5 KiB of LDS per
+// workgroup): the setup is issue-bound and needs the waves to cover its FP64
+// and LDS latencies
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_setup(SegJob* __restrict__ jobs) {
   const SegJob& J = jobs[blockIdx.y];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t pi = rfl(blockIdx.x * 4 + (threadIdx.x >> 6));
   __shared__ uint64_t s_zv[4][kZV];  // per wavefront: z^i of the residue classes
-  __shared__ uint32_t s_zh[4][kZH];
+  __shared__ uint32_t s_zh[4][kZH / 2];
   const uint32_t ngen = J.nprimes, cap = J.cap;
   const uint32_t np = ngen < cap ? ngen : cap;
   if (pi >= np) return;
