@@ -225,17 +225,18 @@ def run_plain(ctx, data, steps, warmup):
                     "threads": prm.threads, "seg_primes": prm.seg_primes}}
 
 
-def model_choice(ctx, base, kcount, nterms, span, log2p, cands, threads_only=0):
-    """The committed cost model's candidate and batch split through the C ABI
-    (rs_model_choose, rs_model_split: the command line's path), or (None,
-    None) without a model; a model without a split head gives split None."""
+def model_choice(ctx, base, kcount, nterms, span, log2p, cands, threads_only=0, k=3):
+    """The committed cost model's k fastest candidates (fastest first) and
+    batch split through the C ABI (rs_model_rank, rs_model_split: the command
+    line's path), or (None, None) without a model; a model without a split
+    head gives split None."""
     from rssieve_amd import tuning
     try:
         cm = tuning.CModel(ctx)
     except R.RSError:
         return None, None
     try:
-        idx = cm.choose(base, kcount, nterms, span, log2p, cands, threads_only)
+        idx = cm.rank(base, kcount, nterms, span, log2p, cands, k, threads_only)
     except R.RSError:
         cm.close()
         return None, None
@@ -247,6 +248,18 @@ def model_choice(ctx, base, kcount, nterms, span, log2p, cands, threads_only=0):
     return idx, split
 
 
+def measured_pick(job, cands, top, pmin, width, log):
+    """The fastest of the model's top candidates, timed on the sample [pmin,
+    pmin + width) of the job (rs_autotune: the job's bitmap and counters are
+    restored), as the command line's -l does."""
+    if len(top) == 1:
+        return top[0]
+    best, secs, primes = job.autotune(pmin, width, [cands[i] for i in top])
+    log["model_top"] = list(top)
+    log["model_top_ns_per_prime"] = [round(1e9 * s / max(n, 1), 3) for s, n in zip(secs, primes)]
+    return top[best]
+
+
 def choose_config(ctx, job, data, log):
     """The tuned configuration and batch split: the committed cost model's
     picks when a model is committed, else the autotuner's measured best on one
@@ -254,9 +267,10 @@ def choose_config(ctx, job, data, log):
     ks, cs, nmin, nmax, bits, wps = data
     cands = ctx.tune_candidates(nmin, nmax, K)
     nterms = int(np.bitwise_count(bits).sum())
-    idx, split = model_choice(ctx, 2, K, nterms, nmax - nmin + 1, np.log2(float(P0)), cands)
-    if idx is not None:
-        how = "cost model"
+    top, split = model_choice(ctx, 2, K, nterms, nmax - nmin + 1, np.log2(float(P0)), cands)
+    if top is not None:
+        how = "cost model top 3, timed"
+        idx = measured_pick(job, cands, top, P0 + PRANGE - 2**31, 2**26, log)
         log["model_pick"] = idx
         log["model_split"] = split
     else:
@@ -460,8 +474,13 @@ def run_high_p(ctx, peak_mad, width=2**28):
     base = 5
     nterms = int(np.bitwise_count(bm).sum())
     cands = ctx.tune_candidates(nmin, nmax, len(ks))
-    idx, split = model_choice(ctx, base, len(ks), nterms, nmax - nmin + 1, 63.0, cands)
+    top, split = model_choice(ctx, base, len(ks), nterms, nmax - nmin + 1, 63.0, cands)
     ref = R.reference_params(nmin, nmax, len(ks), 65536, ctx.compute_units)
+    idx, tlog = None, {}
+    if top is not None:
+        probe = ctx.job(base, ks, cs, nmin, nmax, bm.copy(), wps, ref)
+        idx = measured_pick(probe, cands, top, p0 + 2 * width, 2**25, tlog)  # a sample past the timed range
+        probe.close()
     out = {}
     for label, prm, nv in (("plain", ref, 1), ("tuned", cands[idx if idx is not None else 0], 32)):
         prm.factor_cap = 1 << 16
@@ -499,7 +518,9 @@ def run_high_p(ctx, peak_mad, width=2**28):
             "plain_primes_per_s": pl["primes"] / pl["seconds"], "tuned_primes_per_s": tu["primes"] / tu["seconds"],
             "tuned_over_plain": pl["seconds"] / tu["seconds"] * tu["primes"] / max(pl["primes"], 1),
             "same_terms": same, "tuned_config": {k: tu[k] for k in ("algo", "Q", "threads", "views", "seg_primes")},
-            "candidate": idx, "factors_verified": pl["verified"], "primes": tu["primes"],
+            "candidate": idx, "chosen_by": "cost model top 3, timed" if top is not None else "reference split",
+            "model_top": tlog.get("model_top"), "model_top_ns_per_prime": tlog.get("model_top_ns_per_prime"),
+            "factors_verified": pl["verified"], "primes": tu["primes"],
             "roofline": dict(tu["roof"], bsgs_busy_frac=tu["busy"]),
             "plain_roofline": dict(pl["roof"], bsgs_busy_frac=pl["busy"])}
 

@@ -94,6 +94,13 @@ void rs_model_free(rs_model *m);
  * if none is.  RS_E_ARG if ncand differs from the model's candidate count. */
 int rs_model_choose(rs_model *m, uint32_t base, int32_t kcount, uint64_t nterms, uint32_t span, double log2p,
                     const rs_params *cand, int ncand, uint32_t threads_only, int *idx);
+/* The same job's eligible candidates in order of predicted seconds per prime,
+ * fastest first: the first min(k, eligible) indices into idx[0..k), their
+ * count in *nout (no reference counterpart: a caller times these few on a
+ * sample with rs_autotune instead of all of them, and so recovers most of
+ * the model's error); idx[0] = 0 and *nout = 1 if none is eligible. */
+int rs_model_rank(rs_model *m, uint32_t base, int32_t kcount, uint64_t nterms, uint32_t span, double log2p,
+                  const rs_params *cand, int ncand, uint32_t threads_only, int k, int *idx, int *nout);
 /* The batch split with the lowest predicted seconds per prime for the same
  * job: views per batched round and target primes per segment (the options
  * rs_autotune_split times).  RS_E_ARG if the model has no split head. */

@@ -143,6 +143,9 @@ _SIGS = {
     "rs_model_free": (None, [C.c_void_p]),
     "rs_model_choose": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, C.c_uint64, C.c_uint32, C.c_double,
                                   C.POINTER(Params), C.c_int, C.c_uint32, C.POINTER(C.c_int)]),
+    "rs_model_rank": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, C.c_uint64, C.c_uint32, C.c_double,
+                                C.POINTER(Params), C.c_int, C.c_uint32, C.c_int, C.POINTER(C.c_int),
+                                C.POINTER(C.c_int)]),
     "rs_model_split": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, C.c_uint64, C.c_uint32, C.c_double,
                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "rs_autotune_split": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint32),

@@ -511,10 +511,33 @@ int sieve(Options o) {
       nc = std::min<int>(nc, (int)cand.size());
       const uint32_t span = nmax - nmin + 1;
       const double log2p = std::log2((double)st.p);
-      int idx = 0;
-      if (rs_model_choose(model, base, kcount, nterms, span, log2p, cand.data(), nc, 0, &idx)) {
+      // the model's three fastest candidates, timed on a sample at the start
+      // of the range when it is long enough (about 2^20 primes; the job's
+      // bitmap and counters are restored), the fastest kept
+      int top[3] = {0, 0, 0}, ntop = 1, idx = 0;
+      if (rs_model_rank(model, base, kcount, nterms, span, log2p, cand.data(), nc, 0, 3, top, &ntop)) {
         fprintf(stderr, "warning: %s; using the reference configuration\n", rs_last_error());
-        idx = 0;
+        ntop = 1;
+        top[0] = 0;
+      }
+      idx = top[0];
+      const uint64_t width = (uint64_t)(1048576.0 * std::log((double)st.p));
+      if (ntop > 1 && st.pmax - st.p > 4 * width) {
+        std::vector<rs_params> tc(ntop);
+        for (int i = 0; i < ntop; i++) {
+          tc[i] = cand[top[i]];
+          tc[i].k31_compat = prm.k31_compat;
+          tc[i].factor_cap = prm.factor_cap;
+        }
+        std::vector<double> secs(ntop);
+        std::vector<uint64_t> pr(ntop);
+        int best = 0;
+        if (rs_autotune(r.job, st.p, width, tc.data(), ntop, secs.data(), pr.data(), &best)) die_lib("rs_autotune");
+        idx = top[best];
+        printf("Cost model: candidates %d, %d, %d timed on a sample: %.3g, %.3g, %.3g ns per prime\n", top[0],
+               ntop > 1 ? top[1] : -1, ntop > 2 ? top[2] : -1, 1e9 * secs[0] / (double)std::max<uint64_t>(pr[0], 1),
+               ntop > 1 ? 1e9 * secs[1] / (double)std::max<uint64_t>(pr[1], 1) : 0.0,
+               ntop > 2 ? 1e9 * secs[2] / (double)std::max<uint64_t>(pr[2], 1) : 0.0);
       }
       rs_params pick = cand[idx];
       pick.k31_compat = prm.k31_compat;

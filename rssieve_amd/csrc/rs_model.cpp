@@ -15,6 +15,7 @@
 // log2(views), log2(primes per segment), to that option's log seconds per
 // prime relative to the job's mean (n_in 7, n_out 1), evaluated for every
 // option.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -179,22 +180,37 @@ extern "C" int rs_model_load(rs_ctx* ctx, const char* path, rs_model** out) {
   return RS_OK;
 }
 
-extern "C" int rs_model_choose(rs_model* m, uint32_t base, int32_t kcount, uint64_t nterms, uint32_t span,
-                               double log2p, const rs_params* cand, int ncand, uint32_t threads_only, int* idx) {
-  if (!m || !cand || !idx) return mfail(RS_E_ARG, "rs_model_choose: null argument");
-  if (!m->has_cand) return mfail(RS_E_ARG, "rs_model_choose: the model has no candidate head");
+extern "C" int rs_model_rank(rs_model* m, uint32_t base, int32_t kcount, uint64_t nterms, uint32_t span,
+                             double log2p, const rs_params* cand, int ncand, uint32_t threads_only, int k, int* idx,
+                             int* nout) {
+  if (!m || !cand || !idx || !nout || k < 1) return mfail(RS_E_ARG, "rs_model_rank: null argument or k < 1");
+  if (!m->has_cand) return mfail(RS_E_ARG, "rs_model_rank: the model has no candidate head");
   if ((uint32_t)ncand != m->cand.n_out)
-    return mfail(RS_E_ARG, "rs_model_choose: %d candidates, the model predicts %u", ncand, m->cand.n_out);
+    return mfail(RS_E_ARG, "rs_model_rank: %d candidates, the model predicts %u", ncand, m->cand.n_out);
   std::vector<float> y;
   const int rc = predict(m->cand, false, base, kcount, nterms, span, log2p, &y);
   if (rc) return rc;
-  int best = -1;
-  for (int i = 0; i < ncand; i++) {
-    if (!cand[i].threads || (threads_only && cand[i].threads != threads_only)) continue;
-    if (best < 0 || y[i] < y[best]) best = i;
+  std::vector<int> ok;
+  for (int i = 0; i < ncand; i++)
+    if (cand[i].threads && (!threads_only || cand[i].threads == threads_only)) ok.push_back(i);
+  if (ok.empty()) {  // nothing usable: the reference split (candidate 0)
+    idx[0] = 0;
+    *nout = 1;
+    return RS_OK;
   }
-  *idx = best < 0 ? 0 : best;  // nothing usable: the reference split (candidate 0)
+  // fastest first; equal predictions keep the lower index
+  std::stable_sort(ok.begin(), ok.end(), [&](int a, int b) { return y[a] < y[b]; });
+  const int n = std::min<int>(k, (int)ok.size());
+  for (int i = 0; i < n; i++) idx[i] = ok[i];
+  *nout = n;
   return RS_OK;
+}
+
+extern "C" int rs_model_choose(rs_model* m, uint32_t base, int32_t kcount, uint64_t nterms, uint32_t span,
+                               double log2p, const rs_params* cand, int ncand, uint32_t threads_only, int* idx) {
+  if (!idx) return mfail(RS_E_ARG, "rs_model_choose: null argument");
+  int n = 0;
+  return rs_model_rank(m, base, kcount, nterms, span, log2p, cand, ncand, threads_only, 1, idx, &n);
 }
 
 extern "C" int rs_model_split(rs_model* m, uint32_t base, int32_t kcount, uint64_t nterms, uint32_t span,

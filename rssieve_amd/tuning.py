@@ -378,6 +378,17 @@ class CModel:
                                          C.byref(idx)))
         return idx.value
 
+    def rank(self, base, kcount, nterms, span, log2p, cands, k=3, threads_only=0):
+        """The k eligible candidates with the lowest predictions, fastest first
+        (rs_model_rank)."""
+        C = self._C
+        arr = (R.Params * len(cands))(*cands)
+        idx = (C.c_int * k)()
+        n = C.c_int()
+        R._check(R.lib().rs_model_rank(self._h, base, kcount, nterms, span, log2p, arr, len(cands), threads_only, k,
+                                       idx, C.byref(n)))
+        return [idx[i] for i in range(n.value)]
+
     def split(self, base, kcount, nterms, span, log2p):
         C = self._C
         v, sp = C.c_uint32(), C.c_uint32()

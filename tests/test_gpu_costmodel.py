@@ -73,3 +73,28 @@ def test_split_head_regret(ctx):
         r_model.mean(), ", ".join("%dx%d %.4f" % (v, sp, r) for (v, sp), r in fixed.items())), flush=True)
     assert r_model.mean() < fixed[(1, 65536)] / 5
     assert r_model.mean() < 0.01
+
+
+def test_model_rank_through_the_c_abi(ctx):
+    """rs_model_rank (the committed model through the library): the eligible
+    candidates in order of prediction, the first being rs_model_choose's pick;
+    a thread filter leaves only candidates of that workgroup size, and a
+    candidate list of the wrong length is refused."""
+    import bench
+    cm = tuning.CModel(ctx)
+    try:
+        for nmin, nmax, K, log2p in ((3000000, 9999999, 100, 50.0), (10**6, 2**24 - 1, 100, 63.0),
+                                     (100000, 199999, 10, 30.0)):
+            cands = ctx.tune_candidates(nmin, nmax, K)
+            for threads_only in (0, 256):
+                first = cm.choose(2, K, 10**7, nmax - nmin + 1, log2p, cands, threads_only)
+                top = cm.rank(2, K, 10**7, nmax - nmin + 1, log2p, cands, 3, threads_only)
+                assert top[0] == first and len(set(top)) == len(top) == 3
+                assert all(cands[i].threads and (not threads_only or cands[i].threads == threads_only) for i in top)
+                assert cm.rank(2, K, 10**7, nmax - nmin + 1, log2p, cands, 1, threads_only) == [first]
+        with pytest.raises(R.RSError):
+            cm.rank(2, 100, 10**7, 7 * 10**6, 50.0, cands[:-1], 3)
+        want = cm.rank(2, 100, 10**7, 7 * 10**6, 50.0, cands, 3)
+    finally:
+        cm.close()
+    assert bench.model_choice(ctx, 2, 100, 10**7, 7 * 10**6, 50.0, cands)[0] == want
