@@ -1556,7 +1556,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
   __shared__ uint64_t s_h[2 * RS_MAXK + 24];  // targets of the item (+ the last tile's pad, rg*kU <= 24)
   static_assert(RS_MAXK <= 256, "a sequence index fits the low 8 bits of s_k");
   __shared__ uint32_t s_k[2 * RS_MAXK];   // sequence index | shift (< 2^21) << 8
-  __shared__ uint32_t s_item, s_order, s_dups, s_claims, s_qn;
+  __shared__ uint32_t s_item, s_order, s_dups, s_claims, s_qn, s_wkdone, s_qovf;
   __shared__ uint32_t s_next;  // the block's next item when it is known (same chunk and job), else ~0
   // The fields of the block's current job that every item reads, copied by
   // thread 0 when the job changes (its items are read through these, not
@@ -1661,6 +1661,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
       s_dups = 0;
       s_claims = 0;
       s_qn = 0;
+      s_wkdone = 0;
+      s_qovf = 0;
     }
     __syncthreads();
     // workgroup-uniform values read from LDS go through readfirstlane, so
@@ -1720,7 +1722,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
     // itself holds no table lookups, calls or global accesses.
     auto defer = [&](uint64_t g, uint32_t i, uint32_t qi) {
       const uint32_t slot = atomicAdd(&s_qn, 1u);
-      if (slot < qcap) s_q[slot] = make_uint4((uint32_t)g, (uint32_t)(g >> 32), i, qi);
+      if (slot < qcap) {
+        // the queue's top slots hold the staged W_j until every wavefront has
+        // hashed its share: a positive that would land there earlier makes
+        // the settle redo the steps exactly
+        if (slot < kQCap - 65 || __hip_atomic_load(&s_wkdone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == kW)
+          s_q[slot] = make_uint4((uint32_t)g, (uint32_t)(g >> 32), i, qi);
+        else
+          s_qovf = 1u;
+      }
     };
     // Giant steps in tiles: a tile is a row of rw consecutive giant steps
     // (rw = 64, or 16 or 32 for an item with few giant steps) times rg =
@@ -1871,10 +1881,14 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
     // fills its slots in order, so a search ends at the first bucket with an
     // empty last slot).  Lane 0 of the baby walk writes each W_j's bucket
     // and fingerprint (32 bits; 64j < E) as it passes entry 64j, and W_J's
-    // from W_(J-1) * b^(64*stride) when 64J >= E, to s_wk, which borrows the
-    // positive queue's LDS (idle until the giant steps); after the walk every
-    // thread inserts at most two of them (whash_build, one returning LDS
-    // atomic each, then a barrier before the queue is used).  A fingerprint
+    // from W_(J-1) * b^(64*stride) when 64J >= E, to s_wk, the top 258 words
+    // of the positive queue's LDS (idle until the giant steps); after the
+    // walk's barrier each wavefront inserts its share (whash_build, at most
+    // two per lane, one returning LDS atomic each) and counts itself done in
+    // s_wkdone, with no barrier: the giant steps queue positives from slot 0,
+    // and one that would land in the top slots before all wavefronts are done
+    // (more than 62 positives by then) makes the settle redo the steps
+    // exactly.  A fingerprint
     // hit, rare unless it is a solution, is compared in full with W_j = v0 *
     // (b^(64*stride))^j, computed then, so a hit is exact and the values
     // need no LDS.  Every wavefront takes every kW-th queued
@@ -1887,7 +1901,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
     auto wfp = [](uint64_t w) { return (uint32_t)(w >> 20) & 0x7FFFFFu; };
     constexpr uint32_t kHBBits = kHB == 256 ? 8 : 7;
     static_assert((1u << kHBBits) == kHB && kQCap * 4 >= 258, "s_wk: the bucket and 258 entries in the queue's LDS");
-    uint32_t* const s_wk = (uint32_t*)s_q;
+    uint32_t* const s_wk = (uint32_t*)(s_q + kQCap) - 258;
     auto winfo = [&](uint64_t w) { return (wfp(w) << kHBBits) | (((uint32_t)w >> 2) & (kHB - 1)); };
     // W_j from the walk (lane 0, entry 64j), and W_J after W_(J-1) when 64J >= E
     auto w_put = [&](uint32_t j, uint64_t w) {
@@ -1898,7 +1912,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
                             : mmul(w, s_pw[0][6], p, q));
     };
     auto whash_build = [&]() {
-      for (uint32_t j = tid; j <= nJ; j += T) {
+      for (uint32_t j = wave + kW * lane; j <= nJ; j += T) {
         const uint32_t wi = s_wk[j], entry = ((wi >> kHBBits) << 9) | (j + 1);
         for (uint32_t bk = wi & (kHB - 1);; bk = (bk + 1) & (kHB - 1)) {
           uint32_t k = 0;
@@ -1906,6 +1920,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
           if (k < 4) break;
         }
       }
+      if (lane == 0) atomicAdd(&s_wkdone, 1u);  // this wavefront's reads of s_wk are done
     };
     // W_j in the form the walk gives it (binary64 keys kFpBias + residue, or
     // Montgomery residues)
@@ -1996,7 +2011,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
     auto settle = [&](uint32_t& r0, uint32_t r1, bool final) -> bool {
       __syncthreads();
       const uint32_t nq = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_qn);
-      if (nq > qcap) {
+      if (nq > qcap || rfl(s_qovf)) {
         redo(r0);
         return true;
       }
@@ -2148,13 +2163,18 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
       // ord_p(b) (or its even multiple) is below the table span: the baby
       // values b^(nstart + stride*e) with stride*e < order are distinct and
       // hold every reachable residue (giant.cl:121-134); the targets are
-      // matched against them directly.
-      for (uint32_t i = tid; i < nk; i += T) {
-        const uint64_t h = fp ? (uint64_t)__longlong_as_double((long long)s_h[i]) : s_h[i];
-        s_q[i] = make_uint4((uint32_t)h, (uint32_t)(h >> 32), i, 0u);
+      // matched against them directly, at most kQCap targets (the queue's
+      // size) per pass over the baby values.
+      for (uint32_t c0 = 0; c0 < nk; c0 += kQCap) {
+        const uint32_t nc = min(nk - c0, kQCap);
+        for (uint32_t i = tid; i < nc; i += T) {
+          const uint64_t h = fp ? (uint64_t)__longlong_as_double((long long)s_h[c0 + i]) : s_h[c0 + i];
+          s_q[i] = make_uint4((uint32_t)h, (uint32_t)(h >> 32), c0 + i, 0u);
+        }
+        __syncthreads();
+        match(nc, (order + stride - 1) / stride, order);
+        __syncthreads();  // the claims have read the queue before the next pass refills it
       }
-      __syncthreads();
-      match(nk, (order + stride - 1) / stride, order);
       work = order / stride;  // the baby steps up to the repeat (giant.cl:121-134); no giant products
       probes = nk;
     } else {
@@ -2234,7 +2254,6 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
         }
       };
       whash_build();
-      __syncthreads();  // the hash complete, s_wk read: the queue's LDS is free
       if (fp) {
         GiantFp ar;
         ar.pd = pd;

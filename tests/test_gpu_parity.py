@@ -403,3 +403,50 @@ def test_full_giant_rounds_match_oracle(ctx, case):
     gpu = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, prm)
     orc = oracle_run(base, ks, cs, nmin, nmax, bm, pmin, pmax, nthreads=8)
     _compare(base, ks, cs, nmin, nmax, wps, bm, gpu, orc)
+
+
+@pytest.mark.parametrize("algo", [R.ALGO_MERGED, R.ALGO_CLASSES], ids=["merged", "classes"])
+def test_order_path_with_more_targets_than_the_queue(ctx, algo):
+    """Below p = 32768 the order of b is below a one-list table's span of
+    32768 exponents, so every item takes the order path, which matches its
+    targets against the baby values through the positive queue's LDS.  With
+    100 sequences a one-list item holds up to 200 targets, more than the 128
+    entries of that queue at 256 threads: they are matched in passes.  Base 5,
+    seed 73 gives three primes in [3000, 30000) with 129 to 134 targets (the
+    oracle's setup, counted below).  Bitmap and claims equal the oracle's."""
+    base, K, nmin, nmax, pmin, pmax, seed = 5, 100, 20000, 120000, 3000, 30000, 73
+    ks, cs = _draw(base, K, seed)
+    job = O.make_job(base, ks, cs, nmin, nmax)
+    most = 0
+    for p in O.primes_in_range(pmin, pmax):
+        pr, lst = O.setup_prime(job, int(p), 16384)
+        if pr is not None:
+            most = max(most, sum(2 if par == 1 else 1 for _, par, _ in lst))
+    assert most > 128
+    bm, wps = full_bitmap(K, nmin, nmax, density=0.5, seed=seed)
+    orc = oracle_run(base, ks, cs, nmin, nmax, bm, pmin, pmax)
+    prm = merged_params(ctx, nmin, nmax, K, 16384, 16384, 256, algo=algo)
+    gcnt, gfac, gbm = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, prm)
+    assert gcnt["order_lists"] > 0 and gcnt["seg_overflow"] == 0
+    assert gcnt["claims"] == orc[0]["claims"]
+    assert np.array_equal(gbm, orc[2])
+
+
+def test_early_positives_over_the_staged_hash(ctx):
+    """A filter of 64 words for tables of 64 entries passes a large share of
+    the probes, so an item queues many positives within its first tiles: past
+    slot 62 they would overwrite the W_j staged in the queue's top slots before
+    every wavefront has hashed its share, and such an item, like one whose
+    queue overflows, is redone exactly.  Bitmap and claims equal the oracle's
+    (binary64 and integer paths)."""
+    for base, K, nmin, nmax, pmin, pmax, seed in ((2, 20, 100000, 200000, 2**40, 2**40 + 10**5, 81),
+                                                  (3, 12, 100000, 200000, 2**52, 2**52 + 4 * 10**4, 82)):
+        print("early positives", base, K, pmin, flush=True)
+        ks, cs = _draw(base, K, seed)
+        bm, wps = full_bitmap(K, nmin, nmax, density=0.5, seed=seed)
+        orc = oracle_run(base, ks, cs, nmin, nmax, bm, pmin, pmax)
+        prm = merged_params(ctx, nmin, nmax, K, 64, 64, 256, algo=R.ALGO_CLASSES)
+        gcnt, gfac, gbm = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, prm)
+        assert gcnt["seg_overflow"] == 0
+        assert gcnt["claims"] == orc[0]["claims"]
+        assert np.array_equal(gbm, orc[2])
