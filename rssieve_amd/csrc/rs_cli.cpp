@@ -48,7 +48,8 @@ void help() {
   printf("-f factorfile\t\tOverride default factor file name (psp_sr2sieve.out) with specified file name.\n");
   printf("-d #\t\t\tHIP device index (default 0)\n");
   printf("-t\t\t\tAutotune: time every candidate BSGS configuration on a sample and use the fastest\n");
-  printf("-l\t\t\tLearned: the committed cost model picks the BSGS configuration and the batch split\n");
+  printf("-l\t\t\tLearned: the committed cost model ranks the BSGS configurations (its three fastest\n");
+  printf("\t\t\tare timed on a sample) and picks the batch split\n");
   printf("-m modelfile\t\tLike -l with the given cost model file\n");
   printf("-b #\t\t\tSieve as # views of the range per batched round (one persistent launch for all)\n");
   printf("-c\t\t\tEncode k as the reference's signed int (k >= 2^31 wraps)\n");
