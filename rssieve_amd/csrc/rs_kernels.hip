@@ -1556,7 +1556,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
   __shared__ uint64_t s_h[2 * RS_MAXK + 24];  // targets of the item (+ the last tile's pad, rg*kU <= 24)
   static_assert(RS_MAXK <= 256, "a sequence index fits the low 8 bits of s_k");
   __shared__ uint32_t s_k[2 * RS_MAXK];   // sequence index | shift (< 2^21) << 8
-  __shared__ uint32_t s_item, s_order, s_dups, s_claims, s_qn, s_wkdone, s_qovf;
+  __shared__ uint32_t s_item, s_order, s_qn, s_wkdone, s_qovf;
+  // duplicates and claims of the current item, by item parity (thread 0 sums
+  // one item's while the next item's claims count into the other pair)
+  __shared__ uint32_t s_dups[2], s_claims[2];
   __shared__ uint32_t s_next;  // the block's next item when it is known (same chunk and job), else ~0
   // The fields of the block's current job that every item reads, copied by
   // thread 0 when the job changes (its items are read through these, not
@@ -1622,54 +1625,59 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
     tk = atomicAdd(&bs->work_next, 1u);
     jend = bs->item_base[1];
   }
-  for (;;) {
-    if (tid == 0) {
-      while (ci >= ce && ci < nitems) {  // chunk used up: take the ticket, fetch the next
-        ci = guide_start(gd, tk);
-        ce = guide_start(gd, tk + 1);
-        tk = atomicAdd(&bs->work_next, 1u);
-      }
-      const uint32_t it = ci < ce ? ci++ : nitems;
-      if (it < nitems) {
-        while (it >= jend) jend = bs->item_base[++jcur + 1];
-        if (jcur != jcached) {
-          const SegJob& J = jobs[jcur];
-          const JobDev* jd = J.job;
-          s_jc.job = jd;
-          s_jc.recs = J.recs;
-          s_jc.krecs = J.krecs;
-          s_jc.bitmap = J.bitmap;
-          s_jc.fac = J.fac;
-          s_jc.cnt = J.cnt;
-          s_jc.item_base = bs->item_base[jcur];
-          s_jc.algo = jd->algo;
-          s_jc.hsize = jd->hsize;
-          s_jc.base = jd->base;
-          s_jc.kcount = (uint32_t)jd->kcount;
-          s_jc.nmin = jd->nmin;
-          s_jc.nmax = jd->nmax;
-          s_jc.Q = jd->Q;
-          s_jc.QQ = jd->QQ;
-          s_jc.M = jd->M;
-          s_jc.MM = jd->MM;
-          s_jc.qcap = jd->qcap;
-          jcached = jcur;
-        }
-      }
-      s_item = it;
-      s_next = (ci < ce && ci < jend) ? ci : 0xFFFFFFFFu;
-      s_dups = 0;
-      s_claims = 0;
-      s_qn = 0;
-      s_wkdone = 0;
-      s_qovf = 0;
+  // Thread 0 takes the next item: s_item, s_next and, on a job change, the
+  // job cache.  It runs once before the loop and then during each item, after
+  // the walk's barrier (every wavefront has read the current item's values by
+  // then), so the item's last barrier also publishes the next item and no
+  // barrier of its own is needed.
+  auto fetch = [&]() {
+    while (ci >= ce && ci < nitems) {  // chunk used up: take the ticket, fetch the next
+      ci = guide_start(gd, tk);
+      ce = guide_start(gd, tk + 1);
+      tk = atomicAdd(&bs->work_next, 1u);
     }
-    __syncthreads();
+    const uint32_t it = ci < ce ? ci++ : nitems;
+    if (it < nitems) {
+      while (it >= jend) jend = bs->item_base[++jcur + 1];
+      if (jcur != jcached) {
+        const SegJob& J = jobs[jcur];
+        const JobDev* jd = J.job;
+        s_jc.job = jd;
+        s_jc.recs = J.recs;
+        s_jc.krecs = J.krecs;
+        s_jc.bitmap = J.bitmap;
+        s_jc.fac = J.fac;
+        s_jc.cnt = J.cnt;
+        s_jc.item_base = bs->item_base[jcur];
+        s_jc.algo = jd->algo;
+        s_jc.hsize = jd->hsize;
+        s_jc.base = jd->base;
+        s_jc.kcount = (uint32_t)jd->kcount;
+        s_jc.nmin = jd->nmin;
+        s_jc.nmax = jd->nmax;
+        s_jc.Q = jd->Q;
+        s_jc.QQ = jd->QQ;
+        s_jc.M = jd->M;
+        s_jc.MM = jd->MM;
+        s_jc.qcap = jd->qcap;
+        jcached = jcur;
+      }
+    }
+    s_item = it;
+    s_next = (ci < ce && ci < jend) ? ci : 0xFFFFFFFFu;
+  };
+  if (tid == 0) {
+    fetch();
+    s_dups[0] = s_claims[0] = 0;
+    s_qn = s_wkdone = s_qovf = 0;
+  }
+  __syncthreads();
+  for (uint32_t par = 0;; par ^= 1u) {
     // workgroup-uniform values read from LDS go through readfirstlane, so
     // the compiler keeps the item's values (and the loads that follow from
     // them) in scalar registers
-    // (thread 0 writes them again only after the item's last barrier; an
-    // item without targets passes one barrier of its own first)
+    // (thread 0 writes the next item's only after the walk's barrier; an
+    // item without targets takes it between two barriers of its own)
     const uint32_t item = rfl(s_item);
     if (item >= nitems) break;
     const JobDev* __restrict__ job = rflp(s_jc.job);
@@ -1688,6 +1696,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
     asm volatile("" ::"s"(nk), "s"(nk0), "s"(nk1), "s"(p), "s"(q), "s"(one), "s"(mbase), "s"(rbn), "s"(rbT0),
                  "s"(rbT1), "s"(rg0), "s"(rg1), "s"(rS), "s"(rE), "s"(rM));
     if (nk == 0) {
+      __syncthreads();
+      if (tid == 0) {
+        fetch();
+        s_dups[par ^ 1u] = s_claims[par ^ 1u] = 0;
+      }
       __syncthreads();
       continue;
     }
@@ -1827,9 +1840,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
       const uint32_t kk = s_k[qe.z], kidx = kk & 0xFFu, d = kk >> 8;
       if (ord) {
         for (uint64_t n = (uint64_t)nstart + stride * e + d; n <= nmax; n += ord)
-          claim(job, bitmap, fac, cnt, p, kidx, (uint32_t)n, &s_dups, &s_claims);
+          claim(job, bitmap, fac, cnt, p, kidx, (uint32_t)n, &s_dups[par], &s_claims[par]);
       } else {
-        claim(job, bitmap, fac, cnt, p, kidx, nstart + qe.w * Ql + stride * e + d, &s_dups, &s_claims);
+        claim(job, bitmap, fac, cnt, p, kidx, nstart + qe.w * Ql + stride * e + d, &s_dups[par], &s_claims[par]);
       }
     };
     auto match = [&](uint32_t nq, uint32_t emax, uint32_t ord) __attribute__((always_inline)) {
@@ -1993,7 +2006,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
         if (e >= 0) {
           const uint32_t kk = s_k[i];
           claim(job, bitmap, fac, cnt, p, kk & 0xFFu, nstart + qi * Ql + stride * (uint32_t)e + (kk >> 8),
-                &s_dups, &s_claims);
+                &s_dups[par], &s_claims[par]);
         }
       };
       const uint64_t G = Gm;
@@ -2155,6 +2168,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
       });
     }
     __syncthreads();
+    if (tid == 0) {  // the next item (everyone has read this item's values)
+      fetch();
+      s_dups[par ^ 1u] = s_claims[par ^ 1u] = 0;
+    }
     // the first repeat's exponent offset (s_order holds it minus one), or ~0
     const uint32_t omin = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_order);
     const uint32_t order = omin == 0xFFFFFFFFu ? omin : omin + 1u;
@@ -2277,12 +2294,13 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 256 ? 4 
         acc_flush();
         s_acc_cnt = cnt;
       }
-      s_acc[0] += s_dups;
-      s_acc[1] += s_claims;
+      s_acc[0] += s_dups[par];
+      s_acc[1] += s_claims[par];
       s_acc[2] += work;
       s_acc[3] += probes;
       s_acc[4 + c] += 1;
       if (order != 0xFFFFFFFFu) s_acc[7] += 1;
+      s_qn = s_wkdone = s_qovf = 0;  // the next item's queue (it reaches its own only after its first barrier)
     }
   }
   if (tid == 0) acc_flush();
