@@ -403,6 +403,7 @@ struct MontArith {
   uint64_t p, q, one_, r2;
   __device__ V one() const { return one_; }
   __device__ V neg_one() const { return p - one_; }
+  __device__ V neg(V x) const { return x ? p - x : 0; }  // -x for a canonical x
   __device__ V mul(V a, V b) const { return mmul(a, b, p, q); }
   __device__ V from_int(uint64_t k) const { return mmul(k, r2, p, q); }  // k < 2^64
   __device__ V base(uint32_t b) const {
@@ -441,8 +442,10 @@ struct FpArith {
   // (a sign-extended k in the k >= 2^31 compatibility mode) divide exactly.
   __device__ V from_int(uint64_t k) const {
     if (k >> 53) return (double)(k % (uint64_t)p);
-    return fp_mulmod((double)k, 1.0, p, pinv);
+    const double kd = (double)k;
+    return kd < p ? kd : fp_mulmod(kd, 1.0, p, pinv);  // (k < p, nearly always: k itself)
   }
+  __device__ V neg(V x) const { return x != 0.0 ? p - x : 0.0; }  // -x for a canonical x
   __device__ V base(uint32_t b) const { return from_int(b); }
   __device__ V base_pow(V mb, uint32_t e, uint32_t) const { return fp_powmod_sm(mb, e, p, pinv); }
   __device__ V pow_lg(V x, uint64_t e) const { return fp_powmod_lg(x, e, p, pinv); }
@@ -766,8 +769,8 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
   // the others.
   V h0, h1;
   if (split) {
-    h0 = cp0 ? ar.mul(nmo, mk0) : mk0;
-    h1 = cp1 ? ar.mul(nmo, mk1) : mk1;
+    h0 = cp0 ? ar.neg(mk0) : mk0;
+    h1 = cp1 ? ar.neg(mk1) : mk1;
   } else {
     // batch inversion by inclusive prefix (s) and suffix (u) products over the wavefront
     V s0 = mk0, s1 = mk1, u0 = mk0, u1 = mk1;
@@ -795,8 +798,8 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
     const V inv_k = ar.load(pre.v[kPvInvK]);  // (prod of all k)^-1
     const V kinv0 = ar.mul(inv_k, ar.mul(pre0, suf0));
     const V kinv1 = ar.mul(inv_k, ar.mul(pre1, suf1));
-    h0 = cp0 ? ar.mul(nmo, kinv0) : kinv0;
-    h1 = cp1 ? ar.mul(nmo, kinv1) : kinv1;
+    h0 = cp0 ? ar.neg(kinv0) : kinv0;
+    h1 = cp1 ? ar.neg(kinv1) : kinv1;
   }
 
   // prefilter (setup.cl:59-87); the order of the tests does not change the
