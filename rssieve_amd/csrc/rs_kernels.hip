@@ -87,30 +87,6 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Base-2 strong probable-prime test (getsegprimes.cl:37-66), same shift
-// semantics: N = 2^t + 1 squares 64 times before the final doubling.
-__device__ bool sprp2(uint64_t N) {
-  const uint64_t q = mont_inv(N);
-  const uint64_t one = r_mod(N);
-  const uint64_t nmo = N - one;
-  const uint64_t two = madd(one, one, N);
-  const int t = __ffsll((long long)(N - 1)) - 1;
-  const uint64_t exp = N >> t;
-  uint64_t bit = 0x8000000000000000ull >> ((clz64(exp) + 1) & 63);
-  uint64_t a = two;
-  while (bit) {
-    a = mmul(a, a, N, q);
-    if (exp & bit) a = madd(a, a, N);
-    bit >>= 1;
-  }
-  if (a == one || a == nmo) return true;
-  for (int s = 1; s < t; ++s) {
-    a = mmul(a, a, N, q);
-    if (a == nmo) return true;
-  }
-  return false;
-}
-
 // Start a job's range [pmin, pmax) with a given BSGS workgroup size.
 __global__ void k_set_cursor(SegJob* __restrict__ sj, uint64_t pmin, uint64_t pmax, uint32_t threads, uint64_t range,
                              uint32_t gen_blocks) {
@@ -234,7 +210,7 @@ __global__ __launch_bounds__(256) void k_primegen(SegJob* __restrict__ jobs) {
     bool ok = false;
     if (pos < n) {
       N = cand[pos];
-      ok = sprp2(N);
+      ok = N < kFpLimit ? fp_sprp2(N) : sprp2(N);  // (candidates are odd)
     }
     const uint64_t m = __ballot(ok);
     if (m) {

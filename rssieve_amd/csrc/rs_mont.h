@@ -261,4 +261,57 @@ RS_HD uint64_t r_mod(uint64_t p) {
   return r;
 }
 
+// Base-2 strong probable-prime test (getsegprimes.cl:37-66), same shift
+// semantics: N = 2^t + 1 squares 64 times before the final doubling; on
+// Montgomery residues, for any odd N >= 3.
+RS_HD bool sprp2(uint64_t N) {
+  const uint64_t q = mont_inv(N);
+  const uint64_t one = r_mod(N);
+  const uint64_t nmo = N - one;
+  const uint64_t two = madd(one, one, N);
+  const int t = __builtin_ctzll(N - 1);
+  const uint64_t exp = N >> t;
+  uint64_t bit = 0x8000000000000000ull >> ((clz64(exp) + 1) & 63);
+  uint64_t a = two;
+  while (bit) {
+    a = mmul(a, a, N, q);
+    if (exp & bit) a = madd(a, a, N);
+    bit >>= 1;
+  }
+  if (a == one || a == nmo) return true;
+  for (int s = 1; s < t; ++s) {
+    a = mmul(a, a, N, q);
+    if (a == nmo) return true;
+  }
+  return false;
+}
+
+// The same test on plain residues in binary64 for odd 3 <= N < 2^51: the
+// same steps on the same values (squarings exact by fp_mulmod's bound, doublings of values
+// in (-N, N) folded back into it), compared in canonical form; a fifth of
+// the integer path's instructions per squaring.
+RS_HD bool fp_sprp2(uint64_t N) {
+  const double Nd = (double)N, ninv = 1.0 / Nd;
+  const int t = __builtin_ctzll(N - 1);
+  const uint64_t exp = N >> t;
+  uint64_t bit = 0x8000000000000000ull >> ((clz64(exp) + 1) & 63);
+  double a = 2.0;
+  while (bit) {
+    a = fp_mulmod_signed(a, a, Nd, ninv);
+    if (exp & bit) {
+      a = RS_FADD(a, a);
+      a = a >= Nd ? RS_FADD(a, -Nd) : (a <= -Nd ? RS_FADD(a, Nd) : a);
+    }
+    bit >>= 1;
+  }
+  a = fp_canon(a, Nd);
+  const double nmo = RS_FADD(Nd, -1.0);
+  if (a == 1.0 || a == nmo) return true;
+  for (int s = 1; s < t; ++s) {
+    a = fp_mulmod(a, a, Nd, ninv);
+    if (a == nmo) return true;
+  }
+  return false;
+}
+
 }  // namespace rs

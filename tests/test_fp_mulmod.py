@@ -31,3 +31,18 @@ def test_fp_mulmod_check_detects_moduli_above_limit(checker):
     # sieve keeps those primes on the integer Montgomery path
     r = subprocess.run([checker, "200000", "7", "52"], capture_output=True, text=True)
     assert r.returncode != 0 and "mismatches 0" not in r.stdout
+
+
+def test_fp_sprp2_agrees_with_the_integer_test(tmp_path):
+    """The prime generator's binary64 strong-PRP test below 2^51 gives the
+    integer Montgomery test's answer (the reference's steps) on random odd N,
+    every N = 2^t + 1, known base-2 strong pseudoprimes, every odd N below
+    200001 and the odd N just below 2^51."""
+    exe = str(tmp_path / "fp_sprp_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(HERE, "native", "fp_sprp_check.cpp")],
+                   check=True)
+    r = subprocess.run([exe, "300000", "5"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout
+    assert "mismatches 0" in r.stdout
+    passing = int(r.stdout.split("passing")[1].split()[0])
+    assert passing > 18000  # the primes below 200001 among them
