@@ -1064,6 +1064,7 @@ __device__ __forceinline__ void setup_prime(const A& ar, const SegJob& J, const 
 #pragma unroll
     for (int a = 0; a < 2; a++) {
       const bool va = na > a, vb = nb > a;
+      if (a && __ballot(va || vb) == 0) break;  // no target with a second class on this prime
       const uint32_t da = MS(ra[a] + S - nmod), db = MS(rb[a] + S - nmod);
       const V ta = ar.mul(h0, bdig_pow(da)), tb = ar.mul(h1, bdig_pow(db));
       const uint64_t m0 = __ballot(va), m1 = __ballot(vb);
