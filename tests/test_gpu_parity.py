@@ -450,3 +450,61 @@ def test_early_positives_over_the_staged_hash(ctx):
         assert gcnt["seg_overflow"] == 0
         assert gcnt["claims"] == orc[0]["claims"]
         assert np.array_equal(gbm, orc[2])
+
+
+EDGE_CASES = [
+    # name, base, K, nmin, nmax, pmin, pmax, seed, density
+    ("empty_p_range", 2, 6, 20000, 60000, 10**8, 10**8, 21, 1.0),
+    ("one_number_p_range", 2, 6, 20000, 60000, 10**8 + 7, 10**8 + 8, 22, 1.0),
+    ("every_term_eliminated", 2, 10, 100000, 110000, 10**9, 10**9 + 2 * 10**6, 23, 0.0),
+    ("single_n", 2, 8, 40000, 40000, 10**7, 10**7 + 10**6, 24, 1.0),
+    ("two_n_base3", 3, 8, 40000, 40001, 10**7, 10**7 + 10**6, 25, 1.0),
+]
+
+
+@pytest.mark.parametrize("case", EDGE_CASES, ids=[c[0] for c in EDGE_CASES])
+def test_edge_inputs_match_oracle(ctx, case):
+    """Input edges: an empty p-range (no round runs), a p-range of one number,
+    a bitmap whose terms are all eliminated already (every solution is a
+    duplicate, nothing is claimed), and n-spans of one and two exponents
+    (solutions found past NMAX count as duplicates, giant.cl:21-39).  The
+    reference split is compared counter for counter; the one-list shapes on
+    the cases with a full n-span."""
+    name, base, K, nmin, nmax, pmin, pmax, seed, dens = case
+    print("edge case", case, flush=True)
+    ks, cs = _draw(base, K, seed)
+    bm, wps = full_bitmap(K, nmin, nmax, density=dens, seed=seed)
+    prm = R.reference_params(nmin, nmax, K, 65536, ctx.compute_units)
+    gpu = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax, prm)
+    orc = oracle_run(base, ks, cs, nmin, nmax, bm, pmin, pmax)
+    _compare(base, ks, cs, nmin, nmax, wps, bm, gpu, orc)
+    if pmax == pmin:
+        assert gpu[0]["primecount"] == 0 and np.array_equal(gpu[2], bm)
+    if dens == 0.0:
+        assert gpu[0]["claims"] == 0 and gpu[0]["dups"] > 0 and not gpu[1].size
+    if nmax - nmin >= 10000:
+        for algo in (R.ALGO_MERGED, R.ALGO_CLASSES):
+            print("edge case one-list", name, algo, flush=True)
+            gcnt, gfac, gbm = _run_gpu(ctx, base, ks, cs, nmin, nmax, bm, wps, pmin, pmax,
+                                       merged_params(ctx, nmin, nmax, K, algo=algo))
+            assert gcnt["primecount"] == orc[0]["primecount"] and gcnt["claims"] == orc[0]["claims"]
+            assert np.array_equal(gbm, orc[2])
+
+
+def test_sieve_range_errors(ctx):
+    """pmax below pmin and pmin below 3 are refused before anything runs."""
+    ks, cs = _draw(2, 4, 26)
+    bm, wps = full_bitmap(4, 20000, 30000)
+    job = ctx.job(2, ks, cs, 20000, 30000, bm, wps, R.reference_params(20000, 30000, 4, 65536, ctx.compute_units))
+    try:
+        for lo, hi in ((10**6 + 1, 10**6), (2, 10**6), (0, 0)):
+            with pytest.raises(R.RSError):
+                job.sieve(lo, hi)
+            with pytest.raises(R.RSError):
+                job.sieve_noalloc(lo, hi)
+        with pytest.raises(R.RSError):
+            R.sieve_batch([job], [10**6 + 1], [10**6])
+        assert job.counters().as_dict()["primecount"] == 0
+        assert np.array_equal(job.bitmap(), bm)
+    finally:
+        job.close()
