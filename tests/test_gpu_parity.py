@@ -459,6 +459,10 @@ EDGE_CASES = [
     ("every_term_eliminated", 2, 10, 100000, 110000, 10**9, 10**9 + 2 * 10**6, 23, 0.0),
     ("single_n", 2, 8, 40000, 40000, 10**7, 10**7 + 10**6, 24, 1.0),
     ("two_n_base3", 3, 8, 40000, 40001, 10**7, 10**7 + 10**6, 25, 1.0),
+    # the largest exponents a job takes (NMAX < 2^31) and bases up to 2^32 - 5
+    ("n_below_2_31", 2, 6, 2**31 - 10002, 2**31 - 1, 10**7, 10**7 + 10**6, 27, 1.0),
+    ("base_1000003", 1000003, 6, 20000, 40000, 10**8, 10**8 + 10**6, 28, 1.0),
+    ("base_2_32_minus_5", 2**32 - 5, 6, 20000, 40000, 10**8, 10**8 + 10**6, 29, 1.0),
 ]
 
 
@@ -466,8 +470,9 @@ EDGE_CASES = [
 def test_edge_inputs_match_oracle(ctx, case):
     """Input edges: an empty p-range (no round runs), a p-range of one number,
     a bitmap whose terms are all eliminated already (every solution is a
-    duplicate, nothing is claimed), and n-spans of one and two exponents
-    (solutions found past NMAX count as duplicates, giant.cl:21-39).  The
+    duplicate, nothing is claimed), n-spans of one and two exponents
+    (solutions found past NMAX count as duplicates, giant.cl:21-39), the
+    largest exponents and large bases.  The
     reference split is compared counter for counter; the one-list shapes on
     the cases with a full n-span."""
     name, base, K, nmin, nmax, pmin, pmax, seed, dens = case
